@@ -1,0 +1,231 @@
+"""gsofa_amd -- MI355X fill-path symbolic factorization (Python host mirror).
+
+A thin layer over the C ABI in include/gsofa.h (libgsofa.so).  It mirrors the
+reference's operator surface for this path:
+
+* ``symbfact(A, n_virtual_gpu=..., mem_gb=...)`` stands for the reference's
+  ``symbfact_min_id`` (with_space_optimization/symbfact.cuh:1553): same inputs
+  (colbeg/colend/csr, N_GPU, MEM_GB, max_supernode_size), same fill count, plus
+  the sorted L/U structure and the supernode partition;
+* ``read_ascii`` / ``write_ascii`` handle the reference's three-file format
+  (with_space_optimization/symbfact.cuh:1591-1640);
+* ``Plan`` keeps a matrix resident on the device for repeated execution.
+
+Errors surface as ``GsofaError`` carrying the library status code.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import GSOFA_FLAG_COUNTS_ONLY
+
+__all__ = ["CsrMatrix", "SymbfactResult", "GsofaError", "Plan", "symbfact", "gen_grid",
+           "gen_band", "read_ascii", "write_ascii", "footprint", "GSOFA_FLAG_COUNTS_ONLY"]
+
+
+class GsofaError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        msg = _lib.lib().gsofa_strerror(code).decode()
+        super().__init__("%s%s (status %d)" % (what + ": " if what else "", msg, code))
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        raise GsofaError(rc, what)
+
+
+class CsrMatrix:
+    """Row i's structure is csr[colbeg[i]:colend[i]] (the reference's layout)."""
+
+    def __init__(self, n, colbeg, colend, csr):
+        self.n = int(n)
+        self.colbeg = np.ascontiguousarray(colbeg, dtype=np.int32)
+        self.colend = np.ascontiguousarray(colend, dtype=np.int32)
+        self.csr = np.ascontiguousarray(csr, dtype=np.int32)
+        assert self.colbeg.size == self.n and self.colend.size == self.n
+
+    @property
+    def nnz(self):
+        return int(self.csr.size)
+
+    @classmethod
+    def from_rows(cls, rows):
+        lens = np.array([len(r) for r in rows], dtype=np.int64)
+        end = np.cumsum(lens)
+        beg = end - lens
+        csr = np.concatenate([np.asarray(r, dtype=np.int32) for r in rows]) if rows else np.zeros(0, np.int32)
+        return cls(len(rows), beg, end, csr)
+
+    def rows(self):
+        return [self.csr[b:e] for b, e in zip(self.colbeg, self.colend)]
+
+    def to_c(self):
+        return _lib.Matrix(self.n, self.nnz, self.colbeg.ctypes.data, self.colend.ctypes.data,
+                           self.csr.ctypes.data)
+
+    def fingerprint(self):
+        """Order-independent-of-layout digest of the pattern (rows sorted)."""
+        import hashlib
+        h = hashlib.sha256()
+        h.update(np.int64(self.n).tobytes())
+        for b, e in zip(self.colbeg, self.colend):
+            r = np.sort(self.csr[b:e])
+            h.update(np.int64(r.size).tobytes())
+            h.update(r.astype(np.int32).tobytes())
+        return h.hexdigest()
+
+
+def _from_csr(c):
+    try:
+        n, nnz = int(c.n), int(c.nnz)
+
+        def arr(ptr, cnt):
+            if cnt == 0:
+                return np.zeros(0, np.int32)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_int32)), shape=(cnt,)).copy()
+        return CsrMatrix(n, arr(c.colbeg, n), arr(c.colend, n), arr(c.csr, nnz))
+    finally:
+        _lib.lib().gsofa_csr_free(C.byref(c))
+
+
+def gen_grid(nx, ny, nz=1, nd_leaf=0, extra_prob=0.0, extra_dist=0, seed=0):
+    c = _lib.Csr()
+    _check(_lib.lib().gsofa_gen_grid(nx, ny, nz, nd_leaf, float(extra_prob), extra_dist, seed,
+                                     C.byref(c)), "gen_grid")
+    return _from_csr(c)
+
+
+def gen_band(n, band, lam, seed):
+    c = _lib.Csr()
+    _check(_lib.lib().gsofa_gen_band(n, band, float(lam), seed, C.byref(c)), "gen_band")
+    return _from_csr(c)
+
+
+def read_ascii(colbeg_path, colend_path, csr_path):
+    c = _lib.Csr()
+    _check(_lib.lib().gsofa_read_ascii(colbeg_path.encode(), colend_path.encode(),
+                                       csr_path.encode(), C.byref(c)), "read_ascii")
+    return _from_csr(c)
+
+
+def write_ascii(A, colbeg_path, colend_path, csr_path):
+    m = A.to_c()
+    _check(_lib.lib().gsofa_write_ascii(C.byref(m), colbeg_path.encode(), colend_path.encode(),
+                                        csr_path.encode()), "write_ascii")
+
+
+def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
+             max_supernode_size=0, chunk=0, wave_block_threshold=0, device=0, queue_capacity=0,
+             output_capacity=0, counts_only=False, grid_blocks=0):
+    o = _lib.Options()
+    _lib.lib().gsofa_options_default(C.byref(o))
+    o.n_virtual_gpu = n_virtual_gpu
+    o.virtual_gpu_id = virtual_gpu_id
+    o.mem_budget_bytes = int(mem_budget_bytes or mem_gb * (1 << 30))
+    o.max_supernode_size = max_supernode_size
+    o.chunk = chunk
+    if wave_block_threshold:
+        o.wave_block_threshold = wave_block_threshold
+    o.device = device
+    o.queue_capacity = queue_capacity
+    o.output_capacity = output_capacity
+    o.flags = GSOFA_FLAG_COUNTS_ONLY if counts_only else 0
+    o.grid_blocks = grid_blocks
+    return o
+
+
+class SymbfactResult:
+    """Sorted strictly-lower (L) / strictly-upper (U) structure of L+U by row."""
+
+    def __init__(self, r):
+        n = int(r.n)
+        self.n = n
+        self.fill_count = int(r.fill_count)
+        self.te_alg = int(r.te_alg)
+        self.reach = int(r.reach)
+        self.edge_checks = int(r.edge_checks)
+        self.ms = float(r.ms)
+        self.traverse_ms = float(r.traverse_ms)
+        self.share_ms_min = float(r.share_ms_min)
+        self.share_ms_max = float(r.share_ms_max)
+        self.chunk = int(r.chunk)
+        self.grid_blocks = int(r.grid_blocks)
+        self.queue_capacity = int(r.queue_capacity)
+        self.levels = int(r.levels)
+        self.rescans = int(r.rescans)
+        self.n_super = int(r.n_super)
+
+        def arr(ptr, cnt, ct, dt):
+            if not ptr or cnt <= 0:
+                return np.zeros(max(cnt, 0), dt) if ptr or cnt == 0 else None
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(cnt,)).copy()
+        self.row_fill = arr(r.row_fill, n, C.c_int64, np.int64) if r.row_fill else None
+        self.L_rowptr = arr(r.L_rowptr, n + 1, C.c_int64, np.int64) if r.L_rowptr else None
+        self.U_rowptr = arr(r.U_rowptr, n + 1, C.c_int64, np.int64) if r.U_rowptr else None
+        self.L_col = (arr(r.L_col, int(self.L_rowptr[-1]), C.c_int32, np.int32)
+                      if self.L_rowptr is not None and r.L_col else None)
+        self.U_col = (arr(r.U_col, int(self.U_rowptr[-1]), C.c_int32, np.int32)
+                      if self.U_rowptr is not None and r.U_col else None)
+        if self.n_super >= 0 and r.super_start:
+            self.super_start = arr(r.super_start, self.n_super + 1, C.c_int32, np.int32)
+            self.supno = arr(r.supno, n, C.c_int32, np.int32) if r.supno else np.zeros(0, np.int32)
+        else:
+            self.super_start = None
+            self.supno = None
+
+
+def symbfact(A, **kw):
+    """One-shot symbolic factorization of ``A`` (a CsrMatrix) on the GPU."""
+    m = A.to_c()
+    o = _options(**kw)
+    r = _lib.Result()
+    _check(_lib.lib().gsofa_symbfact(C.byref(m), C.byref(o), C.byref(r)), "symbfact")
+    try:
+        return SymbfactResult(r)
+    finally:
+        _lib.lib().gsofa_result_free(C.byref(r))
+
+
+def footprint(A, **kw):
+    m = A.to_c()
+    o = _options(**kw)
+    return int(_lib.lib().gsofa_plan_footprint(C.byref(m), C.byref(o)))
+
+
+class Plan:
+    """A matrix uploaded once; execute() runs the device path, fetch() copies."""
+
+    def __init__(self, A, **kw):
+        self._A = A  # keep the borrowed arrays alive during create
+        m = A.to_c()
+        o = _options(**kw)
+        h = C.c_void_p()
+        _check(_lib.lib().gsofa_plan_create(C.byref(m), C.byref(o), C.byref(h)), "plan_create")
+        self._h = h
+
+    def execute(self):
+        _check(_lib.lib().gsofa_plan_execute(self._h), "plan_execute")
+
+    def fetch(self):
+        r = _lib.Result()
+        _check(_lib.lib().gsofa_plan_fetch(self._h, C.byref(r)), "plan_fetch")
+        try:
+            return SymbfactResult(r)
+        finally:
+            _lib.lib().gsofa_result_free(C.byref(r))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().gsofa_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,733 @@
+// gsofa_api.hip -- host driver behind the C ABI (include/gsofa.h).
+//
+// Replaces symbfact_min_id (with_space_optimization/symbfact.cuh:1553-2583):
+// the reference's per-level launch + device sync + memcpy loop and its host
+// spill buffers become one cooperative persistent launch per virtual-GPU
+// share, with the level loop, the group loop and queue-overflow recovery on
+// the device.  The host only sizes buffers, launches, and builds outputs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "gsofa.h"
+#include "gsofa_internal.h"
+
+using namespace gsofa;
+
+struct gsofa_plan {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    gsofa_options opt{};
+    int32_t n = 0;
+    int64_t nnz = 0;
+    int32_t* d_rp = nullptr;
+    int32_t* d_ci = nullptr;
+    std::vector<int32_t> h_rp;  // packed row pointers (for sizing and stats)
+
+    int32_t C = 0;
+    uint32_t Q = 0;
+    uint32_t nblocks = 0;
+    int coop = 1;
+    uint64_t lab_words = 0;
+    uint32_t orphan_words = 0;
+    uint32_t* d_lab = nullptr;
+    uint32_t* d_mark = nullptr;
+    uint32_t* d_orphan = nullptr;
+    uint64_t* d_q0 = nullptr;
+    uint64_t* d_q1 = nullptr;
+    uint64_t* d_fills = nullptr;
+    uint64_t fill_cap = 0;
+    Ctrl* d_ctrl = nullptr;
+
+    // persistent label/marker state carried across launches
+    uint32_t off = 0;
+    uint32_t since = 0;
+    uint64_t hw = 0;
+    uint32_t epoch = 0;       // stamp_base = epoch * n
+    bool marks_clean = true;  // markers hold only stamps below the next stamp_base
+
+    // structure outputs
+    uint32_t* d_cntL = nullptr;
+    uint32_t* d_cntU = nullptr;
+    int64_t* d_Lrp = nullptr;
+    int64_t* d_Urp = nullptr;
+    int64_t* d_rowfill = nullptr;
+    int64_t* d_scan = nullptr;
+    size_t scan_words = 0;
+    int32_t* d_Lcol = nullptr;
+    int32_t* d_Ucol = nullptr;
+    int32_t* d_Ltmp = nullptr;
+    int32_t* d_Utmp = nullptr;
+    int64_t Lcap = 0, Ucap = 0;
+    int64_t nnzL = 0, nnzU = 0;
+    int32_t* d_supno = nullptr;
+    int32_t* d_sstart = nullptr;
+    int32_t* d_nsuper = nullptr;
+    int64_t* d_sn64 = nullptr;
+    uint32_t* d_snflag = nullptr;
+
+    // last execute
+    bool executed = false;
+    bool structure = false;
+    bool supernodes = false;
+    Ctrl stats{};
+    double ms = 0, traverse_ms = 0, share_min = 0, share_max = 0;
+    std::vector<hipEvent_t> ev;
+    uint64_t spin_ticks = 0;
+};
+
+namespace {
+
+const char* kErr[] = {"ok",
+                      "invalid argument or malformed matrix",
+                      "allocation failed",
+                      "HIP runtime error",
+                      "problem exceeds 32-bit index limits",
+                      "file could not be read or written",
+                      "no usable GPU",
+                      "does not fit the device-memory budget",
+                      "device-side wait exceeded its bound",
+                      "call not valid in the current state"};
+
+#define HIPCHK(x)                                                                       \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "gsofa: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                \
+            return GSOFA_ERR_HIP;                                                       \
+        }                                                                               \
+    } while (0)
+
+template <class T>
+int dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void**)p, count * sizeof(T)) != hipSuccess) {
+        (void)hipGetLastError();
+        *p = nullptr;
+        return GSOFA_ERR_ALLOC;
+    }
+    return GSOFA_OK;
+}
+
+template <class T>
+void dfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+constexpr int32_t kMaxN = (1 << 25) - 1;  // wave-merge totals stay below 2^31
+
+int validate(const gsofa_matrix* A) {
+    if (!A || A->n < 0 || A->nnz < 0) return GSOFA_ERR_INVALID;
+    if (A->n > kMaxN || A->nnz >= (int64_t)INT32_MAX) return GSOFA_ERR_TOO_LARGE;
+    if (A->n > 0 && (!A->colbeg || !A->colend || (A->nnz > 0 && !A->csr))) return GSOFA_ERR_INVALID;
+    for (int32_t i = 0; i < A->n; i++) {
+        const int64_t b = A->colbeg[i], e = A->colend[i];
+        if (b < 0 || e < b || e > A->nnz) return GSOFA_ERR_INVALID;
+        for (int64_t k = b; k < e; k++)
+            if (A->csr[k] < 0 || A->csr[k] >= A->n) return GSOFA_ERR_INVALID;
+    }
+    return GSOFA_OK;
+}
+
+// Pack rows contiguously, sorted and without duplicates.
+void pack_rows(const gsofa_matrix* A, std::vector<int32_t>& rp, std::vector<int32_t>& ci) {
+    const int32_t n = A->n;
+    rp.assign((size_t)n + 1, 0);
+    ci.resize((size_t)A->nnz);
+    int64_t p = 0;
+    for (int32_t i = 0; i < n; i++) {
+        rp[i] = (int32_t)p;
+        const int64_t b = A->colbeg[i], e = A->colend[i];
+        int32_t* dst = ci.data() + p;
+        std::copy(A->csr + b, A->csr + e, dst);
+        std::sort(dst, dst + (e - b));
+        int64_t len = std::unique(dst, dst + (e - b)) - dst;
+        p += len;
+    }
+    rp[n] = (int32_t)p;
+    ci.resize((size_t)p);
+}
+
+uint64_t share_count(int32_t n, int32_t N, int32_t gid) {
+    return gid < n ? ((uint64_t)n - (uint64_t)gid + (uint64_t)N - 1) / (uint64_t)N : 0;
+}
+
+uint64_t slot_base_h(uint64_t k, uint64_t first, uint64_t N) {
+    return k * (first + 1) + N * (k * (k - 1) / 2);
+}
+
+// Label words the packed arena needs for chunk C (largest group of any share).
+uint64_t label_words(int32_t n, int32_t N, int32_t C, int32_t gid_lo, int32_t gid_hi) {
+    uint64_t best = 0;
+    for (int32_t gid = gid_lo; gid <= gid_hi; gid++) {
+        const uint64_t cnt = share_count(n, N, gid);
+        if (!cnt) continue;
+        const uint64_t groups = (cnt + C - 1) / C;
+        // the last two groups bound the arena (nsrc shrinks only in the last)
+        for (uint64_t g = groups >= 2 ? groups - 2 : 0; g < groups; g++) {
+            const uint64_t k0 = g * (uint64_t)C;
+            const uint64_t nsrc = std::min<uint64_t>((uint64_t)C, cnt - k0);
+            const uint64_t first = (uint64_t)gid + k0 * (uint64_t)N;
+            best = std::max(best, slot_base_h(nsrc, first, N));
+        }
+    }
+    return best;
+}
+
+struct Sizing {
+    int32_t C;
+    uint32_t Q;
+    uint64_t lab_words;
+    uint64_t fill_cap;
+    int64_t work_bytes;  // labels + markers + orphan bitmap + queues
+    int64_t total_bytes;
+};
+
+int64_t work_bytes_for(int32_t n, int32_t N, int32_t C, uint32_t Q, int32_t gid_lo, int32_t gid_hi,
+                       uint64_t* lw) {
+    const uint64_t words = label_words(n, N, C, gid_lo, gid_hi);
+    if (lw) *lw = words;
+    const uint64_t ow = ((uint64_t)n + 31) / 32;
+    return (int64_t)(4 * words + 4 * (uint64_t)C * n + 4 * (uint64_t)C * ow + 16 * (uint64_t)Q);
+}
+
+uint32_t default_queue(int32_t n, int32_t C) {
+    uint64_t q = (uint64_t)C * (uint64_t)n / 2;
+    q = std::max<uint64_t>(q, 4096);
+    q = std::min<uint64_t>(q, 1ull << 28);
+    return (uint32_t)q;
+}
+
+int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_packed,
+                  int64_t avail_bytes, Sizing* sz) {
+    const int32_t n = A->n;
+    const int32_t N = std::max(1, opt->n_virtual_gpu);
+    const int32_t lo = opt->virtual_gpu_id >= 0 ? opt->virtual_gpu_id : 0;
+    const int32_t hi = opt->virtual_gpu_id >= 0 ? opt->virtual_gpu_id : N - 1;
+    const uint64_t maxsrc = share_count(n, N, lo);  // share 0 is the largest
+    int32_t C = opt->chunk > 0 ? opt->chunk : 8192;
+    C = (int32_t)std::min<uint64_t>((uint64_t)C, std::max<uint64_t>(maxsrc, 1));
+    const int64_t budget = opt->mem_budget_bytes > 0 ? opt->mem_budget_bytes : avail_bytes;
+    for (;;) {
+        uint32_t Q = opt->queue_capacity > 0
+                         ? (uint32_t)std::min<int64_t>(opt->queue_capacity, (int64_t)UINT32_MAX - 64)
+                         : default_queue(n, C);
+        uint64_t lw = 0;
+        int64_t wb = work_bytes_for(n, N, C, Q, lo, hi, &lw);
+        if (wb <= budget || C == 1) {
+            if (wb > budget && opt->mem_budget_bytes > 0) {
+                // shrink the queue before giving up
+                while (wb > budget && Q > 64 && opt->queue_capacity <= 0) {
+                    Q /= 2;
+                    wb = work_bytes_for(n, N, C, Q, lo, hi, &lw);
+                }
+                if (wb > budget) return GSOFA_ERR_BUDGET;
+            }
+            sz->C = C;
+            sz->Q = Q;
+            sz->lab_words = lw;
+            sz->work_bytes = wb;
+            uint64_t fc = opt->output_capacity > 0 ? (uint64_t)opt->output_capacity
+                                                   : std::max<uint64_t>(64 * (uint64_t)nnz_packed, 1 << 20);
+            sz->fill_cap = fc;
+            sz->total_bytes = wb + (int64_t)(8 * fc) + 4 * (int64_t)(n + 1 + nnz_packed);
+            return GSOFA_OK;
+        }
+        C = std::max(1, C / 2);
+    }
+}
+
+void advance_state(gsofa_plan* P, int32_t gid) {
+    // replicate the kernel's offset evolution for one launch over a share
+    const uint32_t n = (uint32_t)P->n;
+    const uint32_t N = (uint32_t)std::max(1, P->opt.n_virtual_gpu);
+    const uint32_t loops = 0xFFFFFFFFu / n;
+    const uint64_t cnt = share_count(P->n, (int32_t)N, gid);
+    const uint64_t groups = (cnt + P->C - 1) / P->C;
+    for (uint64_t g = 0; g < groups; g++) {
+        const uint64_t k0 = g * (uint64_t)P->C;
+        const uint64_t nsrc = std::min<uint64_t>((uint64_t)P->C, cnt - k0);
+        const uint64_t size_g = slot_base_h(nsrc, (uint64_t)gid + k0 * N, N);
+        if (P->since >= loops) {
+            P->off = 0xFFFFFFFFu - n;
+            P->since = 0;
+            P->hw = 0;
+        }
+        if (size_g > P->hw) P->hw = size_g;
+        P->off -= n;
+        P->since++;
+    }
+}
+
+void release_device(gsofa_plan* P) {
+    dfree(P->d_rp);
+    dfree(P->d_ci);
+    dfree(P->d_lab);
+    dfree(P->d_mark);
+    dfree(P->d_orphan);
+    dfree(P->d_q0);
+    dfree(P->d_q1);
+    dfree(P->d_fills);
+    dfree(P->d_ctrl);
+    dfree(P->d_cntL);
+    dfree(P->d_cntU);
+    dfree(P->d_Lrp);
+    dfree(P->d_Urp);
+    dfree(P->d_rowfill);
+    dfree(P->d_scan);
+    dfree(P->d_Lcol);
+    dfree(P->d_Ucol);
+    dfree(P->d_Ltmp);
+    dfree(P->d_Utmp);
+    dfree(P->d_supno);
+    dfree(P->d_sstart);
+    dfree(P->d_nsuper);
+    dfree(P->d_sn64);
+    dfree(P->d_snflag);
+}
+
+int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
+    if (nl > P->Lcap) {
+        dfree(P->d_Lcol);
+        dfree(P->d_Ltmp);
+        int64_t cap = nl + nl / 8 + 1024;
+        if (dalloc(&P->d_Lcol, (size_t)cap) || dalloc(&P->d_Ltmp, (size_t)cap)) return GSOFA_ERR_ALLOC;
+        P->Lcap = cap;
+    }
+    if (nu > P->Ucap) {
+        dfree(P->d_Ucol);
+        dfree(P->d_Utmp);
+        int64_t cap = nu + nu / 8 + 1024;
+        if (dalloc(&P->d_Ucol, (size_t)cap) || dalloc(&P->d_Utmp, (size_t)cap)) return GSOFA_ERR_ALLOC;
+        P->Ucap = cap;
+    }
+    return GSOFA_OK;
+}
+
+float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.f;
+    }
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gsofa_strerror(int status) {
+    int i = -status;
+    if (i < 0 || i >= (int)(sizeof(kErr) / sizeof(kErr[0]))) return "unknown status";
+    return kErr[i];
+}
+
+void gsofa_options_default(gsofa_options* o) {
+    if (!o) return;
+    memset(o, 0, sizeof(*o));
+    o->n_virtual_gpu = 1;
+    o->virtual_gpu_id = -1;
+    o->wave_block_threshold = 4096;
+}
+
+void gsofa_result_free(gsofa_result* r) {
+    if (!r) return;
+    free(r->row_fill);
+    free(r->L_rowptr);
+    free(r->L_col);
+    free(r->U_rowptr);
+    free(r->U_col);
+    free(r->super_start);
+    free(r->supno);
+    memset(r, 0, sizeof(*r));
+}
+
+int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
+    gsofa_options o;
+    if (opt) o = *opt;
+    else gsofa_options_default(&o);
+    if (!A || A->n <= 0) return 0;
+    Sizing sz;
+    if (choose_sizing(A, &o, A->nnz, INT64_MAX / 4, &sz) != GSOFA_OK) return -1;
+    return sz.total_bytes;
+}
+
+int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_plan** out) {
+    if (!out) return GSOFA_ERR_INVALID;
+    *out = nullptr;
+    int rc = validate(A);
+    if (rc) return rc;
+    gsofa_plan* P = new gsofa_plan();
+    if (opt) P->opt = *opt;
+    else gsofa_options_default(&P->opt);
+    if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
+    if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu) {
+        delete P;
+        return GSOFA_ERR_INVALID;
+    }
+    if (P->opt.wave_block_threshold <= 0) P->opt.wave_block_threshold = 4096;
+    P->opt.wave_block_threshold = std::min(P->opt.wave_block_threshold, 1 << 20);
+    P->n = A->n;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        delete P;
+        return GSOFA_ERR_NO_DEVICE;
+    }
+    if (P->opt.device < 0 || P->opt.device >= ndev) {
+        delete P;
+        return GSOFA_ERR_INVALID;
+    }
+    P->device = P->opt.device;
+    if (hipSetDevice(P->device) != hipSuccess || hipStreamCreate(&P->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        delete P;
+        return GSOFA_ERR_HIP;
+    }
+    std::vector<int32_t> ci;
+    pack_rows(A, P->h_rp, ci);
+    P->nnz = (int64_t)ci.size();
+    const int32_t n = P->n;
+
+    size_t freeb = 0, totalb = 0;
+    (void)hipMemGetInfo(&freeb, &totalb);
+    const int64_t avail = (int64_t)((double)freeb * 0.80) - 8 * (int64_t)(n + 1 + P->nnz) -
+                          64ll * (n + 1) - (64ll << 20);
+    Sizing sz;
+    rc = choose_sizing(A, &P->opt, P->nnz, std::max<int64_t>(avail, 1 << 20), &sz);
+    if (rc) {
+        (void)hipStreamDestroy(P->stream);
+        delete P;
+        return rc;
+    }
+    P->C = sz.C;
+    P->Q = sz.Q;
+    P->lab_words = sz.lab_words;
+    P->fill_cap = sz.fill_cap;
+    P->orphan_words = (uint32_t)(((uint64_t)n + 31) / 32);
+
+    int e = 0;
+    e |= dalloc(&P->d_rp, (size_t)n + 1);
+    e |= dalloc(&P->d_ci, (size_t)P->nnz);
+    e |= dalloc(&P->d_lab, (size_t)P->lab_words);
+    e |= dalloc(&P->d_mark, (size_t)P->C * (size_t)n);
+    e |= dalloc(&P->d_orphan, (size_t)P->C * P->orphan_words);
+    e |= dalloc(&P->d_q0, (size_t)P->Q);
+    e |= dalloc(&P->d_q1, (size_t)P->Q);
+    e |= dalloc(&P->d_fills, (size_t)P->fill_cap);
+    e |= dalloc(&P->d_ctrl, 1);
+    e |= dalloc(&P->d_cntL, (size_t)n);
+    e |= dalloc(&P->d_cntU, (size_t)n);
+    e |= dalloc(&P->d_Lrp, (size_t)n + 1);
+    e |= dalloc(&P->d_Urp, (size_t)n + 1);
+    e |= dalloc(&P->d_rowfill, (size_t)n);
+    P->scan_words = scan_tmp_words(n) + 2 * (size_t)n + 8;
+    e |= dalloc(&P->d_scan, P->scan_words);
+    if (e) {
+        release_device(P);
+        (void)hipStreamDestroy(P->stream);
+        delete P;
+        return GSOFA_ERR_ALLOC;
+    }
+    bool ok = hipMemcpyAsync(P->d_rp, P->h_rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
+                             P->stream) == hipSuccess;
+    if (P->nnz)
+        ok = ok && hipMemcpyAsync(P->d_ci, ci.data(), sizeof(int32_t) * P->nnz, hipMemcpyHostToDevice,
+                                  P->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * (size_t)P->C * P->orphan_words,
+                              P->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(P->stream) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        release_device(P);
+        (void)hipStreamDestroy(P->stream);
+        delete P;
+        return GSOFA_ERR_HIP;
+    }
+    P->off = 0xFFFFFFFFu - (uint32_t)std::max(n, 1);
+    P->since = 0;
+    P->hw = 0;
+    P->epoch = 0;
+
+    // persistent grid: as many blocks as can be co-resident
+    int occ = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, P->device) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, traverse_kernel_ptr(), traverse_block_size(),
+                                                     0) != hipSuccess ||
+        occ <= 0) {
+        (void)hipGetLastError();
+        release_device(P);
+        (void)hipStreamDestroy(P->stream);
+        delete P;
+        return GSOFA_ERR_HIP;
+    }
+    uint32_t maxblocks = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
+    P->nblocks = P->opt.grid_blocks > 0 ? std::min<uint32_t>((uint32_t)P->opt.grid_blocks, maxblocks)
+                                         : maxblocks;
+    int rate_khz = 0;
+    if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, P->device) != hipSuccess ||
+        rate_khz <= 0) {
+        (void)hipGetLastError();
+        rate_khz = 100000;
+    }
+    P->spin_ticks = (uint64_t)rate_khz * 20000ull;  // 20 s
+    *out = P;
+    return GSOFA_OK;
+}
+
+int gsofa_plan_execute(gsofa_plan* P) {
+    if (!P) return GSOFA_ERR_INVALID;
+    HIPCHK(hipSetDevice(P->device));
+    const int32_t n = P->n;
+    const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    std::vector<int32_t> shares;
+    if (P->opt.virtual_gpu_id >= 0) shares.push_back(P->opt.virtual_gpu_id);
+    else
+        for (int32_t g = 0; g < N; g++) shares.push_back(g);
+    const bool all_rows = P->opt.virtual_gpu_id < 0 || N == 1;
+    P->structure = !(P->opt.flags & GSOFA_FLAG_COUNTS_ONLY);
+    P->supernodes = P->structure && all_rows && n > 0;
+    P->executed = false;
+    memset(&P->stats, 0, sizeof(P->stats));
+    if (n == 0) {
+        P->executed = true;
+        P->nnzL = P->nnzU = 0;
+        return GSOFA_OK;
+    }
+    const size_t nev = 2 * shares.size() + 2;
+    while (P->ev.size() < nev) {
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        P->ev.push_back(e);
+    }
+
+    for (int attempt = 0; attempt < 3; attempt++) {
+        HIPCHK(hipMemsetAsync(P->d_ctrl, 0, sizeof(Ctrl), P->stream));
+        HIPCHK(hipEventRecord(P->ev[0], P->stream));
+        for (size_t si = 0; si < shares.size(); si++) {
+            const int32_t gid = shares[si];
+            const uint64_t cnt = share_count(n, N, gid);
+            // stamps must increase per marker slot: move to a fresh stamp range
+            // (or clear the markers when the 32-bit range is used up)
+            if (!P->marks_clean) {
+                if ((uint64_t)(P->epoch + 2) * (uint64_t)n >= 0xFFFFFFFFull) {
+                    HIPCHK(hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream));
+                    P->epoch = 0;
+                } else {
+                    P->epoch++;
+                }
+            }
+            P->marks_clean = false;
+            TravParams tp{};
+            tp.rp = P->d_rp;
+            tp.ci = P->d_ci;
+            tp.n = n;
+            tp.n_gpu = N;
+            tp.gpu_id = gid;
+            tp.chunk = P->C;
+            tp.n_groups = (int32_t)((cnt + P->C - 1) / P->C);
+            tp.first_group = 0;
+            tp.lab = P->d_lab;
+            tp.lab_words = P->lab_words;
+            tp.mark = P->d_mark;
+            tp.qbuf0 = P->d_q0;
+            tp.qbuf1 = P->d_q1;
+            tp.qcap = P->Q;
+            tp.orphan_words = P->orphan_words;
+            tp.orphan = P->d_orphan;
+            tp.fills = P->d_fills;
+            tp.fill_cap = P->fill_cap;
+            tp.ctrl = P->d_ctrl;
+            tp.nblocks = P->nblocks;
+            tp.block_threshold = P->opt.wave_block_threshold;
+            tp.off_start = P->off;
+            tp.groups_since_init = P->since;
+            tp.lab_hw = P->hw;
+            tp.stamp_base = P->epoch * (uint32_t)n;
+            tp.spin_ticks = P->spin_ticks;
+            HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+            if (tp.n_groups > 0) HIPCHK(launch_traverse(tp, P->stream, P->coop));
+            HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+            advance_state(P, gid);
+        }
+        HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
+        if (!P->structure || P->stats.fill_pos <= P->fill_cap) break;
+        // fill stream outgrew its reserve: grow to the exact size and rerun
+        dfree(P->d_fills);
+        P->fill_cap = P->stats.fill_pos + P->stats.fill_pos / 16 + 1024;
+        if (dalloc(&P->d_fills, (size_t)P->fill_cap)) return GSOFA_ERR_ALLOC;
+    }
+    if (P->structure && P->stats.fill_pos > P->fill_cap) return GSOFA_ERR_ALLOC;
+    P->traverse_ms = 0;
+    P->share_min = 1e300;
+    P->share_max = 0;
+    for (size_t si = 0; si < shares.size(); si++) {
+        double t = ev_ms(P->ev[2 + 2 * si], P->ev[3 + 2 * si]);
+        P->traverse_ms += t;
+        P->share_min = std::min(P->share_min, t);
+        P->share_max = std::max(P->share_max, t);
+    }
+
+    if (P->structure) {
+        FinalParams f{};
+        f.n = n;
+        f.rp = P->d_rp;
+        f.ci = P->d_ci;
+        f.fills = P->d_fills;
+        f.n_fills = P->stats.fill_pos;
+        f.n_gpu = N;
+        f.gpu_id = all_rows ? -1 : P->opt.virtual_gpu_id;
+        f.L_rowptr = P->d_Lrp;
+        f.U_rowptr = P->d_Urp;
+        f.row_fill = P->d_rowfill;
+        f.cntL = P->d_cntL;
+        f.cntU = P->d_cntU;
+        f.scan_tmp = P->d_scan;
+        HIPCHK(launch_count(f, P->stream));
+        HIPCHK(exclusive_scan_u32_to_i64(P->d_cntL, P->d_Lrp, n, P->d_scan, P->stream));
+        HIPCHK(exclusive_scan_u32_to_i64(P->d_cntU, P->d_Urp, n, P->d_scan, P->stream));
+        int64_t tot[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        P->nnzL = tot[0];
+        P->nnzU = tot[1];
+        int rc = ensure_cols(P, P->nnzL, P->nnzU);
+        if (rc) return rc;
+        f.L_col = P->d_Ltmp;
+        f.U_col = P->d_Utmp;
+        HIPCHK(launch_scatter(f, P->stream));
+        HIPCHK(launch_sort_segments(P->d_Lrp, P->d_Ltmp, P->d_Lcol, n, P->stream));
+        HIPCHK(launch_sort_segments(P->d_Urp, P->d_Utmp, P->d_Ucol, n, P->stream));
+        if (P->supernodes) {
+            if (!P->d_supno) {
+                int e = 0;
+                e |= dalloc(&P->d_supno, (size_t)n);
+                e |= dalloc(&P->d_sstart, (size_t)n + 1);
+                e |= dalloc(&P->d_nsuper, 1);
+                e |= dalloc(&P->d_sn64, scan_tmp_words(n) + 2 * (size_t)n + 8);
+                e |= dalloc(&P->d_snflag, (size_t)n);
+                if (e) return GSOFA_ERR_ALLOC;
+            }
+            SuperParams sp{};
+            sp.n = n;
+            sp.L_rowptr = P->d_Lrp;
+            sp.L_col = P->d_Lcol;
+            sp.U_rowptr = P->d_Urp;
+            sp.max_size = P->opt.max_supernode_size;
+            sp.supno = P->d_supno;
+            sp.super_start = P->d_sstart;
+            sp.n_super = P->d_nsuper;
+            sp.tmp64 = P->d_sn64;
+            sp.flag = P->d_snflag;
+            HIPCHK(launch_supernodes(sp, P->stream));
+        }
+    }
+    HIPCHK(hipEventRecord(P->ev[1], P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    P->ms = ev_ms(P->ev[0], P->ev[1]);
+    P->executed = true;
+    return GSOFA_OK;
+}
+
+int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
+    if (!P || !r) return GSOFA_ERR_INVALID;
+    if (!P->executed) return GSOFA_ERR_STATE;
+    memset(r, 0, sizeof(*r));
+    const int32_t n = P->n;
+    r->n = n;
+    r->fill_count = P->stats.fill_count;
+    r->te_alg = P->stats.te;
+    r->reach = P->stats.reach;
+    r->edge_checks = P->stats.edge_checks;
+    r->levels = P->stats.levels;
+    r->rescans = P->stats.rescans;
+    r->ms = P->ms;
+    r->traverse_ms = P->traverse_ms;
+    r->share_ms_min = P->share_min;
+    r->share_ms_max = P->share_max;
+    r->chunk = P->C;
+    r->grid_blocks = (int32_t)P->nblocks;
+    r->queue_capacity = P->Q;
+    r->n_super = -1;
+    if (!P->structure || n == 0) {
+        if (n == 0) {
+            r->L_rowptr = (int64_t*)calloc(1, sizeof(int64_t));
+            r->U_rowptr = (int64_t*)calloc(1, sizeof(int64_t));
+            r->n_super = 0;
+            r->super_start = (int32_t*)calloc(1, sizeof(int32_t));
+        }
+        return GSOFA_OK;
+    }
+    HIPCHK(hipSetDevice(P->device));
+    r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
+    r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(P->nnzL, 1));
+    r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(P->nnzU, 1));
+    if (!r->row_fill || !r->L_rowptr || !r->U_rowptr || !r->L_col || !r->U_col) {
+        gsofa_result_free(r);
+        return GSOFA_ERR_ALLOC;
+    }
+    HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
+    if (P->nnzL)
+        HIPCHK(hipMemcpyAsync(r->L_col, P->d_Lcol, sizeof(int32_t) * P->nnzL, hipMemcpyDeviceToHost, P->stream));
+    if (P->nnzU)
+        HIPCHK(hipMemcpyAsync(r->U_col, P->d_Ucol, sizeof(int32_t) * P->nnzU, hipMemcpyDeviceToHost, P->stream));
+    if (P->supernodes) {
+        int32_t ns = 0;
+        HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        r->n_super = ns;
+        r->super_start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ns + 1));
+        r->supno = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+        if (!r->super_start || !r->supno) {
+            gsofa_result_free(r);
+            return GSOFA_ERR_ALLOC;
+        }
+        HIPCHK(hipMemcpyAsync(r->super_start, P->d_sstart, sizeof(int32_t) * (ns + 1), hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipMemcpyAsync(r->supno, P->d_supno, sizeof(int32_t) * n, hipMemcpyDeviceToHost, P->stream));
+    }
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return GSOFA_OK;
+}
+
+void gsofa_plan_destroy(gsofa_plan* P) {
+    if (!P) return;
+    (void)hipSetDevice(P->device);
+    (void)hipStreamSynchronize(P->stream);
+    for (auto e : P->ev) (void)hipEventDestroy(e);
+    release_device(P);
+    (void)hipStreamDestroy(P->stream);
+    delete P;
+}
+
+int gsofa_symbfact(const gsofa_matrix* A, const gsofa_options* opt, gsofa_result* out) {
+    if (!out) return GSOFA_ERR_INVALID;
+    memset(out, 0, sizeof(*out));
+    gsofa_plan* P = nullptr;
+    int rc = gsofa_plan_create(A, opt, &P);
+    if (rc) return rc;
+    rc = gsofa_plan_execute(P);
+    if (!rc) rc = gsofa_plan_fetch(P, out);
+    gsofa_plan_destroy(P);
+    return rc;
+}
+
+}  // extern "C"

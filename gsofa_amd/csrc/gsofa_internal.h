@@ -1,0 +1,111 @@
+// gsofa_internal.h -- structures shared by the host driver and the CDNA4
+// kernels.  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace gsofa {
+
+// Device control block.  Zeroed by the host before every traversal launch.
+struct Ctrl {
+    // generation barrier
+    uint32_t bar_count;
+    uint32_t bar_gen;
+    uint32_t abort;        // set when a bounded device wait gives up
+    uint32_t pad0;
+    // rotating per-level counters (level L reads slot L%3)
+    unsigned long long qcnt[3];   // queue sizes (pairs appended, may exceed capacity)
+    uint32_t cursor[4];           // work-fetch cursors
+    unsigned long long orphans;   // frontier pairs parked in the overflow bitmap
+    unsigned long long fill_pos;  // fill append position
+    // statistics, accumulated once per block at kernel end
+    unsigned long long fill_count;
+    unsigned long long te;
+    unsigned long long reach;
+    unsigned long long edge_checks;
+    unsigned long long levels;
+    unsigned long long rescans;
+};
+
+struct TravParams {
+    const int32_t* rp;     // n+1 packed row pointers
+    const int32_t* ci;     // nnz column indices, rows contiguous
+    int32_t n;
+    int32_t n_gpu;         // virtual GPU count (source stride)
+    int32_t gpu_id;        // share being processed
+    int32_t chunk;         // concurrent sources per group (C)
+    int32_t n_groups;
+    int32_t first_group;   // groups [first_group, first_group+n_groups) of this launch
+    uint32_t* lab;         // packed max-id label arena
+    uint64_t lab_words;
+    uint32_t* mark;        // C x n fill markers (stamp = source + 1)
+    uint64_t* qbuf0;       // frontier pair queues: (slot << 32) | vertex
+    uint64_t* qbuf1;
+    uint32_t qcap;
+    uint32_t orphan_words; // ceil(n/32)
+    uint32_t* orphan;      // C x orphan_words overflow bitmap
+    uint64_t* fills;       // appended fills: (row << 32) | col
+    uint64_t fill_cap;
+    Ctrl* ctrl;
+    uint32_t nblocks;      // persistent grid size
+    int32_t block_threshold; // rows at least this long are expanded by a whole block
+    uint32_t off_start;    // max-id offset of the first group of this launch
+    uint32_t groups_since_init; // groups already run since the last label re-init
+    uint64_t lab_hw;       // label words already initialised before this launch
+    uint32_t stamp_base;   // marker stamp of source s is stamp_base + s + 1
+    uint32_t pad1;
+    uint64_t spin_ticks;   // wall-clock ticks a barrier waits before aborting
+};
+
+// Launchers (gsofa_kernels.hip)
+hipError_t launch_traverse(const TravParams& p, hipStream_t s, int cooperative);
+int traverse_block_size();
+const void* traverse_kernel_ptr();
+size_t traverse_lds_bytes();
+
+hipError_t launch_fill(uint32_t* p, uint64_t count, uint32_t value, hipStream_t s);
+hipError_t launch_pack_rows(const int32_t* colbeg, const int32_t* colend, const int32_t* csr,
+                            const int32_t* rp, int32_t* ci, int32_t n, hipStream_t s);
+
+struct FinalParams {
+    int32_t n;
+    const int32_t* rp;
+    const int32_t* ci;
+    const uint64_t* fills;
+    uint64_t n_fills;
+    int32_t n_gpu;
+    int32_t gpu_id;        // -1: all rows processed
+    int64_t* L_rowptr;     // n+1
+    int64_t* U_rowptr;     // n+1
+    int32_t* L_col;
+    int32_t* U_col;
+    int64_t* row_fill;     // n
+    uint32_t* cntL;        // n scratch (counts, then cursors)
+    uint32_t* cntU;
+    int64_t* scan_tmp;     // scratch for scans
+};
+hipError_t launch_count(const FinalParams& f, hipStream_t s);
+hipError_t launch_scatter(const FinalParams& f, hipStream_t s);
+hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n,
+                                hipStream_t s);
+
+// Inclusive/exclusive scans over int64 (used for row pointers and supernodes).
+size_t scan_tmp_words(int64_t n);
+hipError_t exclusive_scan_u32_to_i64(const uint32_t* in, int64_t* out, int64_t n,
+                                     int64_t* tmp, hipStream_t s);  // out has n+1 entries
+
+struct SuperParams {
+    int32_t n;
+    const int64_t* L_rowptr;
+    const int32_t* L_col;
+    const int64_t* U_rowptr;
+    int32_t max_size;
+    int32_t* supno;        // n
+    int32_t* super_start;  // n+1
+    int32_t* n_super;      // 1
+    int64_t* tmp64;        // scratch: 2n+1 + scan_tmp_words(n)
+    uint32_t* flag;        // n scratch
+};
+hipError_t launch_supernodes(const SuperParams& p, hipStream_t s);
+
+}  // namespace gsofa

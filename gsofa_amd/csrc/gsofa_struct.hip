@@ -1,0 +1,411 @@
+// gsofa_struct.hip -- device kernels that turn the traversal's fill stream into
+// sorted per-row L and U structure, plus the supernode pass and small helpers
+// (scans, fills, CSR repacking).
+//
+// The reference never materialises L/U (its markers are transient,
+// with_space_optimization/symbfact.cuh:2126) and has only a stub for
+// supernodes (with_space_optimization/with_supernodes/symbfact.cuh:1087-1101);
+// the outputs here are defined by the fill-path sets themselves.
+#include "gsofa_internal.h"
+
+namespace gsofa {
+
+namespace {
+
+constexpr int SB = 256;          // threads per block for the helpers
+constexpr int SCAN_ITEMS = 8;    // items per thread in a scan block
+constexpr int SCAN_TILE = SB * SCAN_ITEMS;
+constexpr int SORT_LDS = 4096;   // longest segment sorted by LDS bitonic
+constexpr int BM_WORDS = 8192;   // 32 KB LDS bitmap for long segments
+constexpr uint32_t BM_BITS = BM_WORDS * 32u;
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Repack rows so that row i occupies ci[rp[i] .. rp[i+1]) (rp precomputed).
+__global__ void k_pack_rows(const int32_t* colbeg, const int32_t* colend, const int32_t* csr,
+                            const int32_t* rp, int32_t* ci, int32_t n) {
+    const int lane = threadIdx.x & 63;
+    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t i = w; i < n; i += nw) {
+        const int32_t b = colbeg[i], e = colend[i], o = rp[i];
+        for (int32_t k = b + lane; k < e; k += 64) ci[o + (k - b)] = csr[k];
+    }
+}
+
+__device__ __forceinline__ bool row_processed(int32_t s, int32_t n_gpu, int32_t gpu_id) {
+    return gpu_id < 0 || (s % n_gpu) == gpu_id;
+}
+
+// Per processed row: A's strictly-lower / strictly-upper entry counts.
+__global__ void k_count_rows(FinalParams f) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < f.n;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t l = 0, u = 0;
+        const bool proc = row_processed((int32_t)s, f.n_gpu, f.gpu_id);
+        if (proc) {
+            for (int32_t k = f.rp[s]; k < f.rp[s + 1]; k++) {
+                const int32_t j = f.ci[k];
+                l += (j < s);
+                u += (j > s);
+            }
+        }
+        f.cntL[s] = l;
+        f.cntU[s] = u;
+        f.row_fill[s] = proc ? 0 : -1;
+    }
+}
+
+__global__ void k_count_fills(FinalParams f) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n_fills;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = f.fills[i];
+        const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
+        atomicAdd(col < row ? &f.cntL[row] : &f.cntU[row], 1u);
+        atomicAdd((unsigned long long*)&f.row_fill[row], 1ull);
+    }
+}
+
+// A's off-diagonal entries of each processed row go first (one thread per row,
+// so no atomics); cursors start at zero.
+__global__ void k_scatter_rows(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < f.n;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t l = 0, u = 0;
+        if (row_processed((int32_t)s, f.n_gpu, f.gpu_id)) {
+            const int64_t pl = f.L_rowptr[s], pu = f.U_rowptr[s];
+            for (int32_t k = f.rp[s]; k < f.rp[s + 1]; k++) {
+                const int32_t j = f.ci[k];
+                if (j < s) tmpL[pl + l++] = j;
+                else if (j > s) tmpU[pu + u++] = j;
+            }
+        }
+        f.cntL[s] = l;
+        f.cntU[s] = u;
+    }
+}
+
+__global__ void k_scatter_fills(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n_fills;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t e = f.fills[i];
+        const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
+        if (col < row) tmpL[f.L_rowptr[row] + atomicAdd(&f.cntL[row], 1u)] = (int32_t)col;
+        else tmpU[f.U_rowptr[row] + atomicAdd(&f.cntU[row], 1u)] = (int32_t)col;
+    }
+}
+
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t x, uint32_t* sh, uint32_t* total) {
+    // sh: SB words
+    const int t = threadIdx.x;
+    sh[t] = x;
+    __syncthreads();
+    for (int d = 1; d < SB; d <<= 1) {
+        uint32_t y = t >= d ? sh[t - d] : 0u;
+        __syncthreads();
+        sh[t] += y;
+        __syncthreads();
+    }
+    uint32_t incl = sh[t];
+    *total = sh[SB - 1];
+    __syncthreads();
+    return incl - x;
+}
+
+// Sort each row segment (L part and U part separately) from tmp into the final
+// arrays.  Values within a segment are distinct, so a segment longer than the
+// LDS bitonic tile is sorted by setting bits of an LDS bitmap over successive
+// value windows and emitting them in order.
+__global__ __launch_bounds__(SB) void k_sort_segments(const int64_t* rowptr, const int32_t* src,
+                                                      int32_t* dst, int32_t n) {
+    __shared__ int32_t s[SORT_LDS];
+    __shared__ uint32_t bm[BM_WORDS];
+    __shared__ uint32_t sh[SB];
+    __shared__ int32_t red[2];
+    const int t = threadIdx.x;
+    for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
+        const int64_t b = rowptr[r], e = rowptr[r + 1];
+        const int64_t len = e - b;
+        if (len <= 0) continue;
+        if (len == 1) {
+            if (t == 0) dst[b] = src[b];
+            continue;
+        }
+        if (len <= SORT_LDS) {
+            int P = 2;
+            while (P < len) P <<= 1;
+            for (int i = t; i < P; i += SB) s[i] = i < len ? src[b + i] : INT32_MAX;
+            __syncthreads();
+            for (int k = 2; k <= P; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    for (int i = t; i < P; i += SB) {
+                        const int ixj = i ^ j;
+                        if (ixj > i) {
+                            const int32_t a = s[i], c = s[ixj];
+                            const bool up = (i & k) == 0;
+                            if ((a > c) == up) {
+                                s[i] = c;
+                                s[ixj] = a;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = t; i < len; i += SB) dst[b + i] = s[i];
+            __syncthreads();
+            continue;
+        }
+        // long segment: value range, then bitmap windows
+        if (t == 0) {
+            red[0] = INT32_MAX;
+            red[1] = INT32_MIN;
+        }
+        __syncthreads();
+        int32_t lo = INT32_MAX, hi = INT32_MIN;
+        for (int64_t i = t; i < len; i += SB) {
+            const int32_t v = src[b + i];
+            lo = v < lo ? v : lo;
+            hi = v > hi ? v : hi;
+        }
+        atomicMin(&red[0], lo);
+        atomicMax(&red[1], hi);
+        __syncthreads();
+        lo = red[0];
+        hi = red[1];
+        int64_t written = 0;
+        for (int64_t wb = lo; wb <= hi; wb += BM_BITS) {
+            for (int i = t; i < BM_WORDS; i += SB) bm[i] = 0u;
+            __syncthreads();
+            for (int64_t i = t; i < len; i += SB) {
+                const int64_t v = src[b + i];
+                if (v >= wb && v < wb + (int64_t)BM_BITS) {
+                    const uint32_t o = (uint32_t)(v - wb);
+                    atomicOr(&bm[o >> 5], 1u << (o & 31u));
+                }
+            }
+            __syncthreads();
+            constexpr int WPT = BM_WORDS / SB;  // words per thread
+            uint32_t cnt = 0;
+            for (int w = 0; w < WPT; w++) cnt += (uint32_t)__popc(bm[t * WPT + w]);
+            uint32_t total;
+            uint32_t pos = block_excl_scan_u32(cnt, sh, &total);
+            int64_t o = b + written + pos;
+            for (int w = 0; w < WPT; w++) {
+                uint32_t bits = bm[t * WPT + w];
+                while (bits) {
+                    const uint32_t bp = (uint32_t)__ffs(bits) - 1u;
+                    bits &= bits - 1u;
+                    dst[o++] = (int32_t)(wb + (int64_t)(t * WPT + w) * 32 + bp);
+                }
+            }
+            written += total;
+            __syncthreads();
+        }
+    }
+}
+
+// ---- scans -----------------------------------------------------------------
+struct OpAdd {
+    __device__ static int64_t apply(int64_t a, int64_t b) { return a + b; }
+    __device__ static int64_t identity() { return 0; }
+};
+struct OpMax {
+    __device__ static int64_t apply(int64_t a, int64_t b) { return a > b ? a : b; }
+    __device__ static int64_t identity() { return INT64_MIN; }
+};
+
+template <class Op>
+__global__ __launch_bounds__(SB) void k_scan_tiles(int64_t* a, int64_t len, int64_t* sums) {
+    __shared__ int64_t sh[SB];
+    const int t = threadIdx.x;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)t * SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    int64_t acc = Op::identity();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int64_t idx = base + i;
+        v[i] = idx < len ? a[idx] : Op::identity();
+        acc = Op::apply(acc, v[i]);
+        v[i] = acc;
+    }
+    sh[t] = acc;
+    __syncthreads();
+    for (int d = 1; d < SB; d <<= 1) {
+        int64_t y = t >= d ? sh[t - d] : Op::identity();
+        __syncthreads();
+        sh[t] = Op::apply(sh[t], y);
+        __syncthreads();
+    }
+    const int64_t prefix = t > 0 ? sh[t - 1] : Op::identity();
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int64_t idx = base + i;
+        if (idx < len) a[idx] = Op::apply(prefix, v[i]);
+    }
+    if (t == SB - 1 && sums) sums[blockIdx.x] = sh[SB - 1];
+}
+
+template <class Op>
+__global__ void k_scan_add(int64_t* a, int64_t len, const int64_t* sums) {
+    const int64_t tile = blockIdx.x + 1;
+    const int64_t add = sums[tile - 1];
+    for (int64_t i = tile * SCAN_TILE + threadIdx.x; i < (tile + 1) * SCAN_TILE && i < len; i += SB)
+        a[i] = Op::apply(add, a[i]);
+}
+
+template <class Op>
+hipError_t inclusive_scan(int64_t* a, int64_t len, int64_t* tmp, hipStream_t s) {
+    if (len <= 0) return hipSuccess;
+    const int64_t tiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    hipLaunchKernelGGL(k_scan_tiles<Op>, dim3((uint32_t)tiles), dim3(SB), 0, s, a, len,
+                       tiles > 1 ? tmp : nullptr);
+    if (tiles > 1) {
+        hipError_t e = inclusive_scan<Op>(tmp, tiles, tmp + tiles, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_scan_add<Op>, dim3((uint32_t)(tiles - 1)), dim3(SB), 0, s, a, len, tmp);
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_u32_to_i64_shift(const uint32_t* in, int64_t* out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i + 1] = in[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = 0;
+}
+
+// ---- supernodes --------------------------------------------------------------
+// Row i joins row i-1's supernode iff (i-1) is the last entry of L_i (i.e. the
+// traversal from i passes through i-1), |U_i| = |U_{i-1}| - 1, and the
+// supernode holds fewer than max_size rows.  The cap splits a maximal run of
+// joining rows into consecutive pieces of max_size rows.
+__global__ void k_sn_flags(SuperParams p, int64_t* key) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        bool join = false;
+        if (i > 0) {
+            const int64_t lb = p.L_rowptr[i], le = p.L_rowptr[i + 1];
+            const bool pass = le > lb && p.L_col[le - 1] == (int32_t)(i - 1);
+            const int64_t ui = p.U_rowptr[i + 1] - p.U_rowptr[i];
+            const int64_t up = p.U_rowptr[i] - p.U_rowptr[i - 1];
+            join = pass && (ui == up - 1);
+        }
+        p.flag[i] = join ? 1u : 0u;
+        key[i] = join ? 0 : i;  // run start via inclusive max-scan
+    }
+}
+
+__global__ void k_sn_starts(SuperParams p, const int64_t* runstart, int64_t* cnt) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const bool join = p.flag[i] != 0u;
+        const int64_t pos = i - runstart[i];
+        const bool start = !join || (p.max_size > 0 && (pos % p.max_size) == 0);
+        p.flag[i] = start ? 1u : 0u;
+        cnt[i + 1] = start ? 1 : 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[0] = 0;
+}
+
+__global__ void k_sn_write(SuperParams p, const int64_t* excl) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t id = excl[i] + (int64_t)p.flag[i] - 1;
+        p.supno[i] = (int32_t)id;
+        if (p.flag[i]) p.super_start[excl[i]] = (int32_t)i;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int32_t ns = (int32_t)excl[p.n];
+        *p.n_super = ns;
+        p.super_start[ns] = p.n;
+    }
+}
+
+inline dim3 grid_for(int64_t work, int per = SB, int cap = 65536) {
+    int64_t g = (work + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return dim3((uint32_t)g);
+}
+
+}  // namespace
+
+hipError_t launch_fill(uint32_t* p, uint64_t count, uint32_t value, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, grid_for((int64_t)count), dim3(SB), 0, s, p, count, value);
+    return hipGetLastError();
+}
+
+hipError_t launch_pack_rows(const int32_t* colbeg, const int32_t* colend, const int32_t* csr,
+                            const int32_t* rp, int32_t* ci, int32_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pack_rows, grid_for((int64_t)n * 64), dim3(SB), 0, s, colbeg, colend, csr,
+                       rp, ci, n);
+    return hipGetLastError();
+}
+
+size_t scan_tmp_words(int64_t n) {
+    size_t words = 0;
+    int64_t len = n + 1;
+    while (len > 1) {
+        len = (len + SCAN_TILE - 1) / SCAN_TILE;
+        words += (size_t)len;
+    }
+    return words + 16;
+}
+
+hipError_t exclusive_scan_u32_to_i64(const uint32_t* in, int64_t* out, int64_t n, int64_t* tmp,
+                                     hipStream_t s) {
+    hipLaunchKernelGGL(k_u32_to_i64_shift, grid_for(n), dim3(SB), 0, s, in, out, n);
+    return inclusive_scan<OpAdd>(out + 1, n, tmp, s);
+}
+
+hipError_t launch_count(const FinalParams& f, hipStream_t s) {
+    hipLaunchKernelGGL(k_count_rows, grid_for(f.n), dim3(SB), 0, s, f);
+    if (f.n_fills) hipLaunchKernelGGL(k_count_fills, grid_for((int64_t)f.n_fills), dim3(SB), 0, s, f);
+    return hipGetLastError();
+}
+
+// tmp columns live in L_col/U_col's twin buffers passed through FinalParams:
+// the caller passes the final arrays in L_col/U_col and scratch via cntL-sized
+// buffers; see gsofa_api.hip for the buffer rotation.
+hipError_t launch_scatter(const FinalParams& f, hipStream_t s) {
+    // f.L_col / f.U_col here are the *unsorted* staging arrays
+    hipLaunchKernelGGL(k_scatter_rows, grid_for(f.n), dim3(SB), 0, s, f, f.L_col, f.U_col);
+    if (f.n_fills)
+        hipLaunchKernelGGL(k_scatter_fills, grid_for((int64_t)f.n_fills), dim3(SB), 0, s, f, f.L_col,
+                           f.U_col);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n,
+                                hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_segments, grid_for(n, 1, 1 << 20), dim3(SB), 0, s, rowptr, src, dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_supernodes(const SuperParams& p, hipStream_t s) {
+    if (p.n <= 0) {
+        hipLaunchKernelGGL(k_sn_write, dim3(1), dim3(SB), 0, s, p, p.tmp64);
+        return hipGetLastError();
+    }
+    int64_t* key = p.tmp64;             // n
+    int64_t* cnt = p.tmp64 + p.n;       // n+1
+    int64_t* scratch = cnt + p.n + 1;   // scan temporaries
+    hipLaunchKernelGGL(k_sn_flags, grid_for(p.n), dim3(SB), 0, s, p, key);
+    hipError_t e = inclusive_scan<OpMax>(key, p.n, scratch, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sn_starts, grid_for(p.n), dim3(SB), 0, s, p, key, cnt);
+    e = inclusive_scan<OpAdd>(cnt + 1, p.n, scratch, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_sn_write, grid_for(p.n), dim3(SB), 0, s, p, cnt);
+    return hipGetLastError();
+}
+
+}  // namespace gsofa

@@ -1,0 +1,166 @@
+/*
+ * gsofa.h -- C ABI of the MI355X fill-path symbolic factorization engine.
+ *
+ * Drop-in boundary for the reference's single entry point
+ *     void symbfact_min_id(int argc, char** argv, int myrank,
+ *                          ull_t& fill_count, double& time)
+ * (with_space_optimization/symbfact.cuh:1553, called from
+ *  with_space_optimization/main.cu:31), which reads three ASCII files
+ * (with_space_optimization/symbfact.cuh:1591-1640), runs the multi-source
+ * fill-path traversal for every virtual GPU share
+ * (with_space_optimization/symbfact.cuh:1995-2462) and prints
+ * "Total Number of fill-ins detected: N" (with_space_optimization/symbfact.cuh:2526).
+ *
+ * Here the same work is split into plain C entry points with status codes
+ * (never exit()), borrowed read-only inputs and library-owned outputs.
+ * Threading: one handle (plan) is used by one host thread at a time.
+ */
+#ifndef GSOFA_H
+#define GSOFA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference prints and calls exit(); barrier.cuh:11-20,
+ *      symbfact.cuh:1559,1592-1597) ---------------------------------------- */
+#define GSOFA_OK 0
+#define GSOFA_ERR_INVALID -1      /* bad argument or malformed matrix          */
+#define GSOFA_ERR_ALLOC -2        /* host or device allocation failed          */
+#define GSOFA_ERR_HIP -3          /* a HIP runtime call failed                 */
+#define GSOFA_ERR_TOO_LARGE -4    /* exceeds 32-bit index limits               */
+#define GSOFA_ERR_IO -5           /* file could not be read or written         */
+#define GSOFA_ERR_NO_DEVICE -6    /* no usable GPU                             */
+#define GSOFA_ERR_BUDGET -7       /* does not fit the device-memory budget     */
+#define GSOFA_ERR_DEVICE_TIMEOUT -8 /* a device-side wait exceeded its bound   */
+#define GSOFA_ERR_STATE -9        /* call not valid in the object's state      */
+
+const char* gsofa_strerror(int status);
+
+/* ---- input ------------------------------------------------------------------
+ * Borrowed, read-only, the reference's layout: row i's structure (an edge
+ * i -> j for every j listed) is csr[colbeg[i] .. colend[i]).  Rows may be
+ * unsorted and colbeg need not be monotone (with_space_optimization/dataset/).
+ * Entries must be distinct within a row; the diagonal is expected present. */
+typedef struct gsofa_matrix {
+    int32_t n;
+    int64_t nnz; /* length of csr; must be < 2^31 */
+    const int32_t* colbeg;
+    const int32_t* colend;
+    const int32_t* csr;
+} gsofa_matrix;
+
+/* Library-owned matrix (generators, file reader); free with gsofa_csr_free. */
+typedef struct gsofa_csr {
+    int32_t n;
+    int64_t nnz;
+    int32_t* colbeg;
+    int32_t* colend;
+    int32_t* csr;
+} gsofa_csr;
+void gsofa_csr_free(gsofa_csr* m);
+
+/* ---- options ---------------------------------------------------------------
+ * n_virtual_gpu / mem_budget_bytes are the reference CLI's N_GPU and MEM_GB
+ * (with_space_optimization/symbfact.cuh:1565,1571); max_supernode_size is the
+ * supernode variant's extra knob (with_space_optimization/with_supernodes/
+ * symbfact.cuh:1577).  The remaining fields replace compile-time #defines. */
+#define GSOFA_FLAG_COUNTS_ONLY 0x1  /* fill counts only: no L/U or supernodes  */
+
+typedef struct gsofa_options {
+    int32_t n_virtual_gpu;        /* sources split cyclically into this many shares (>=1) */
+    int32_t virtual_gpu_id;       /* share to run, or -1 for every share in turn          */
+    int64_t mem_budget_bytes;     /* labels+markers+queues pool; 0 = automatic             */
+    int32_t max_supernode_size;   /* 0 = unbounded                                          */
+    int32_t chunk;                /* concurrent sources per group; 0 = automatic            */
+    int32_t wave_block_threshold; /* row length from which a whole workgroup expands a row  */
+    int32_t device;               /* HIP device ordinal                                     */
+    int64_t queue_capacity;       /* frontier pairs per queue; 0 = automatic                */
+    int64_t output_capacity;      /* fill entries reserved; 0 = automatic (grown on demand) */
+    int32_t flags;                /* GSOFA_FLAG_*                                            */
+    int32_t grid_blocks;          /* persistent grid size; 0 = occupancy-bounded maximum    */
+} gsofa_options;
+
+void gsofa_options_default(gsofa_options* opt);
+
+/* ---- result ----------------------------------------------------------------
+ * Library-owned; release with gsofa_result_free.  L is the strictly lower and
+ * U the strictly upper part of struct(L+U), sorted by column within each row.
+ * Rows outside the processed share are empty and have row_fill = -1.
+ * fill_count = |struct(L+U)| - |struct(A)| over processed rows, diagonal
+ * excluded (the reference's "fill-ins"). */
+typedef struct gsofa_result {
+    int32_t n;
+    uint64_t fill_count;
+    uint64_t te_alg;      /* sum_s [deg(s) + sum_{v in R_<(s)} deg(v)] over processed rows */
+    uint64_t reach;       /* sum_s |R_<(s)|                                                */
+    uint64_t edge_checks; /* edges actually examined (>= te_alg; re-expansions included)  */
+    int64_t* row_fill;    /* n */
+    int64_t* L_rowptr;    /* n+1 */
+    int32_t* L_col;
+    int64_t* U_rowptr;    /* n+1 */
+    int32_t* U_col;
+    int32_t n_super;      /* -1 when not computed (single share or counts only)      */
+    int32_t* super_start; /* n_super+1 entries, last = n                              */
+    int32_t* supno;       /* n entries: supernode id of each row                       */
+    double ms;            /* device time of traversal + structure + supernodes         */
+    double traverse_ms;   /* device time of the traversal kernel(s) alone              */
+    double share_ms_min;  /* fastest / slowest share (reference log, symbfact.cuh:2530) */
+    double share_ms_max;
+    int32_t chunk;        /* concurrent sources per group actually used                */
+    int32_t grid_blocks;  /* persistent grid actually used                             */
+    int64_t queue_capacity;
+    uint64_t levels;      /* frontier levels summed over groups                        */
+    uint64_t rescans;     /* queue-overflow drain-and-resume passes                    */
+} gsofa_result;
+
+void gsofa_result_free(gsofa_result* r);
+
+/* One-shot: upload, run, fetch, release.  Replaces symbfact_min_id
+ * (with_space_optimization/symbfact.cuh:1553). */
+int gsofa_symbfact(const gsofa_matrix* A, const gsofa_options* opt, gsofa_result* out);
+
+/* ---- device-resident plan ---------------------------------------------------
+ * create: validate, upload the CSR and size the workspace (no traversal).
+ * execute: traversal + L/U structure + supernodes on the device; the inputs
+ *          stay resident and results stay on the device.
+ * fetch:   copy the last execute's results to a gsofa_result.  */
+typedef struct gsofa_plan gsofa_plan;
+int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_plan** plan);
+int gsofa_plan_execute(gsofa_plan* plan);
+int gsofa_plan_fetch(gsofa_plan* plan, gsofa_result* out);
+void gsofa_plan_destroy(gsofa_plan* plan);
+/* Device bytes a plan for A would hold (CSR + workspace + output reserve). */
+int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt);
+
+/* ---- inputs: reference file format and generators ---------------------------
+ * ASCII triple (with_space_optimization/symbfact.cuh:1591-1640): the csr file
+ * holds nnz, n and nnz indices; colbeg/colend hold n integers each. */
+int gsofa_read_ascii(const char* colbeg_path, const char* colend_path, const char* csr_path,
+                     gsofa_csr* out);
+int gsofa_write_ascii(const gsofa_matrix* A, const char* colbeg_path, const char* colend_path,
+                      const char* csr_path);
+
+/* 5-point (nz == 1) or 7-point Laplacian pattern on an nx*ny*nz grid.  nd_leaf
+ * > 0 orders it by geometric nested dissection with that leaf size (0 =
+ * natural row-major).  With probability extra_prob a row gains one extra
+ * one-directional entry to a uniformly chosen grid point at Manhattan distance
+ * 2..extra_dist.  Rows are sorted. */
+int gsofa_gen_grid(int32_t nx, int32_t ny, int32_t nz, int32_t nd_leaf, double extra_prob,
+                   int32_t extra_dist, uint64_t seed, gsofa_csr* out);
+/* Diagonal plus Poisson(lambda) off-diagonal columns per row drawn uniformly
+ * from |i-j| <= band (j != i), duplicates removed.  Rows are sorted. */
+int gsofa_gen_band(int32_t n, int32_t band, double lambda, uint64_t seed, gsofa_csr* out);
+/* Deterministic scalar draws used by the batch / stream workloads. */
+int32_t gsofa_gen_side_loguniform(uint64_t seed, uint64_t index, int32_t lo, int32_t hi);
+int32_t gsofa_gen_uniform_int(uint64_t seed, uint64_t purpose, uint64_t index, int32_t lo,
+                              int32_t hi);
+double gsofa_gen_unit(uint64_t seed, uint64_t purpose, uint64_t index);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSOFA_H */

@@ -1,0 +1,55 @@
+"""Compute oracle goldens for the large workloads and write them as JSON.
+
+Run in this repository (CPU only):  python tests/golden/make_goldens.py [c2] [c3] [c4]
+
+For each workload the CPU oracle (oracle/liboracle.so) produces the full
+structure; the fixture keeps its totals and SHA-256 digests of the sorted L/U
+arrays and of the supernode partitions, so the GPU parity tests can check
+full-size results without rerunning the oracle.  Digest layout is defined in
+tests/digests.py.
+"""
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import gsofa_amd.configs as cf  # noqa: E402
+from oracle import oracle  # noqa: E402
+from digests import oracle_record  # noqa: E402
+
+
+def run(name, A, out):
+    t = time.time()
+    r = oracle.symbfact(A.colbeg, A.colend, A.csr, A.n)
+    rec = oracle_record(A, r)
+    rec["oracle_seconds"] = round(time.time() - t, 1)
+    out[name] = rec
+    print(name, json.dumps(rec), flush=True)
+
+
+def main(which):
+    path = os.path.join(HERE, "goldens.json")
+    out = json.load(open(path)) if os.path.exists(path) else {}
+    if "c2" in which:
+        run("c2", cf.config2(), out)
+    if "c4" in which:
+        recs = []
+        for b, A in enumerate(cf.config4()):
+            r = oracle.symbfact(A.colbeg, A.colend, A.csr, A.n)
+            rec = oracle_record(A, r)
+            recs.append(rec)
+        out["c4"] = recs
+        print("c4", len(recs), flush=True)
+    if "c3" in which:
+        run("c3", cf.config3(), out)
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:] or ["c2", "c4", "c3"])
