@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgsofa.so")
 
 GSOFA_FLAG_COUNTS_ONLY = 0x1
+MODE_AUTO, MODE_BLOCK, MODE_GRID = 0, 1, 2
 
 # Every entry point include/gsofa.h declares (kept in sync by a CPU test).
 EXPORTS = [
@@ -37,7 +38,7 @@ class Options(C.Structure):
                 ("mem_budget_bytes", C.c_int64), ("max_supernode_size", C.c_int32),
                 ("chunk", C.c_int32), ("wave_block_threshold", C.c_int32), ("device", C.c_int32),
                 ("queue_capacity", C.c_int64), ("output_capacity", C.c_int64),
-                ("flags", C.c_int32), ("grid_blocks", C.c_int32)]
+                ("flags", C.c_int32), ("grid_blocks", C.c_int32), ("mode", C.c_int32)]
 
 
 class Result(C.Structure):
@@ -47,7 +48,8 @@ class Result(C.Structure):
                 ("U_rowptr", C.c_void_p), ("U_col", C.c_void_p), ("n_super", C.c_int32),
                 ("super_start", C.c_void_p), ("supno", C.c_void_p), ("ms", C.c_double),
                 ("traverse_ms", C.c_double), ("share_ms_min", C.c_double),
-                ("share_ms_max", C.c_double), ("chunk", C.c_int32), ("grid_blocks", C.c_int32),
+                ("share_ms_max", C.c_double), ("chunk", C.c_int32), ("mode", C.c_int32),
+                ("grid_blocks", C.c_int32),
                 ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64)]
 
 

@@ -29,7 +29,13 @@ struct gsofa_plan {
     int32_t* d_ci = nullptr;
     std::vector<int32_t> h_rp;  // packed row pointers (for sizing and stats)
 
-    int32_t C = 0;
+    int mode = GSOFA_MODE_BLOCK;
+    int32_t C = 0;        // grid mode: sources per group; block mode: nblocks * S
+    uint32_t S = 0;       // block mode: slots per workgroup
+    uint32_t ql_cap = 0, qg_cap = 0;
+    size_t blk_lds = 0;
+    uint64_t* d_qg = nullptr;
+    uint32_t* d_bstate = nullptr;
     uint32_t Q = 0;
     uint32_t nblocks = 0;
     int coop = 1;
@@ -245,6 +251,65 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
     }
 }
 
+// Block mode: LDS per workgroup for two workgroups per CU, slots per
+// workgroup from the option or 16, shrunk until the slot arrays fit.
+int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
+    const int32_t n = P->n;
+    if (P->opt.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, P->opt.mem_budget_bytes);
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, blk_kernel_ptr()) != hipSuccess) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
+    int max_blk = 0;
+    if (hipDeviceGetAttribute(&max_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, P->device) != hipSuccess)
+        max_blk = 65536;
+    const int64_t per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
+                                                                     : 65536;
+    int64_t target = std::min<int64_t>(max_blk, per_cu / 2);
+    int64_t dyn = target - (int64_t)fa.sharedSizeBytes - 512;
+    uint32_t ql = dyn > 16 * 64 ? (uint32_t)(dyn / 16) : 64;
+    uint32_t qg = 1u << 16;
+    if (P->opt.queue_capacity > 0) {
+        const uint64_t qc = (uint64_t)std::min<int64_t>(P->opt.queue_capacity, 1ll << 30);
+        ql = (uint32_t)std::min<uint64_t>(ql, qc);
+        qg = (uint32_t)(qc - ql);
+    }
+    P->ql_cap = ql;
+    P->qg_cap = qg;
+    P->blk_lds = (size_t)ql * 16;
+    if (P->blk_lds > 65536)
+        (void)hipFuncSetAttribute(blk_kernel_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)P->blk_lds);
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(), blk_block_size(),
+                                                     P->blk_lds) != hipSuccess || occ <= 0) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
+    uint32_t nb = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
+    if (P->opt.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)P->opt.grid_blocks);
+    const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    const uint64_t maxsrc = std::max<uint64_t>(share_count(n, N, 0), 1);
+    uint32_t S = P->opt.chunk > 0 ? (uint32_t)std::max<int64_t>(1, P->opt.chunk / (int64_t)nb) : 16u;
+    S = std::min<uint32_t>(S, (uint32_t)kMaxSlots);
+    // no more slots than sources
+    while (S > 1 && (uint64_t)nb * (S / 2) >= maxsrc) S /= 2;
+    if ((uint64_t)nb > maxsrc && S == 1) nb = (uint32_t)std::max<uint64_t>(maxsrc, 1);
+    const uint64_t ow = ((uint64_t)n + 31) / 32;
+    auto bytes = [&](uint32_t nbk, uint32_t s) {
+        return (int64_t)((uint64_t)nbk * s * (8ull * n + 4 * ow) + (uint64_t)nbk * 2 * qg * 8 + 8ull * nbk);
+    };
+    while (bytes(nb, S) > avail && S > 1) S /= 2;
+    while (bytes(nb, S) > avail && nb > 1) nb /= 2;
+    if (bytes(nb, S) > avail) return GSOFA_ERR_BUDGET;
+    P->nblocks = nb;
+    P->S = S;
+    P->C = (int32_t)(nb * S);
+    P->Q = ql + qg;
+    return GSOFA_OK;
+}
+
 void advance_state(gsofa_plan* P, int32_t gid) {
     // replicate the kernel's offset evolution for one launch over a share
     const uint32_t n = (uint32_t)P->n;
@@ -292,6 +357,8 @@ void release_device(gsofa_plan* P) {
     dfree(P->d_nsuper);
     dfree(P->d_sn64);
     dfree(P->d_snflag);
+    dfree(P->d_qg);
+    dfree(P->d_bstate);
 }
 
 int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
@@ -402,27 +469,51 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
     (void)hipMemGetInfo(&freeb, &totalb);
     const int64_t avail = (int64_t)((double)freeb * 0.80) - 8 * (int64_t)(n + 1 + P->nnz) -
                           64ll * (n + 1) - (64ll << 20);
-    Sizing sz;
-    rc = choose_sizing(A, &P->opt, P->nnz, std::max<int64_t>(avail, 1 << 20), &sz);
-    if (rc) {
+    P->mode = P->opt.mode == GSOFA_MODE_GRID ? GSOFA_MODE_GRID : GSOFA_MODE_BLOCK;
+    P->orphan_words = (uint32_t)(((uint64_t)n + 31) / 32);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, P->device) != hipSuccess) {
+        (void)hipGetLastError();
         (void)hipStreamDestroy(P->stream);
         delete P;
-        return rc;
+        return GSOFA_ERR_HIP;
     }
-    P->C = sz.C;
-    P->Q = sz.Q;
-    P->lab_words = sz.lab_words;
-    P->fill_cap = sz.fill_cap;
-    P->orphan_words = (uint32_t)(((uint64_t)n + 31) / 32);
-
     int e = 0;
+    if (P->mode == GSOFA_MODE_BLOCK) {
+        rc = size_block_mode(P, prop, std::max<int64_t>(avail, 1 << 20));
+        if (rc) {
+            (void)hipStreamDestroy(P->stream);
+            delete P;
+            return rc;
+        }
+        const size_t slots = (size_t)P->nblocks * P->S;
+        e |= dalloc(&P->d_lab, slots * (size_t)n);
+        e |= dalloc(&P->d_mark, slots * (size_t)n);
+        e |= dalloc(&P->d_orphan, slots * P->orphan_words);
+        e |= dalloc(&P->d_qg, (size_t)P->nblocks * 2 * std::max<uint32_t>(P->qg_cap, 1));
+        e |= dalloc(&P->d_bstate, 2 * (size_t)P->nblocks);
+        P->lab_words = slots * (size_t)n;
+    } else {
+        Sizing sz;
+        rc = choose_sizing(A, &P->opt, P->nnz, std::max<int64_t>(avail, 1 << 20), &sz);
+        if (rc) {
+            (void)hipStreamDestroy(P->stream);
+            delete P;
+            return rc;
+        }
+        P->C = sz.C;
+        P->Q = sz.Q;
+        P->lab_words = sz.lab_words;
+        e |= dalloc(&P->d_lab, (size_t)P->lab_words);
+        e |= dalloc(&P->d_mark, (size_t)P->C * (size_t)n);
+        e |= dalloc(&P->d_orphan, (size_t)P->C * P->orphan_words);
+        e |= dalloc(&P->d_q0, (size_t)P->Q);
+        e |= dalloc(&P->d_q1, (size_t)P->Q);
+    }
+    P->fill_cap = P->opt.output_capacity > 0 ? (uint64_t)P->opt.output_capacity
+                                             : std::max<uint64_t>(64 * (uint64_t)P->nnz, 1 << 20);
     e |= dalloc(&P->d_rp, (size_t)n + 1);
     e |= dalloc(&P->d_ci, (size_t)P->nnz);
-    e |= dalloc(&P->d_lab, (size_t)P->lab_words);
-    e |= dalloc(&P->d_mark, (size_t)P->C * (size_t)n);
-    e |= dalloc(&P->d_orphan, (size_t)P->C * P->orphan_words);
-    e |= dalloc(&P->d_q0, (size_t)P->Q);
-    e |= dalloc(&P->d_q1, (size_t)P->Q);
     e |= dalloc(&P->d_fills, (size_t)P->fill_cap);
     e |= dalloc(&P->d_ctrl, 1);
     e |= dalloc(&P->d_cntL, (size_t)n);
@@ -438,14 +529,28 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
         delete P;
         return GSOFA_ERR_ALLOC;
     }
+    const size_t mark_words = P->mode == GSOFA_MODE_BLOCK ? (size_t)P->nblocks * P->S * n
+                                                          : (size_t)P->C * n;
+    const size_t orph_words = P->mode == GSOFA_MODE_BLOCK ? (size_t)P->nblocks * P->S * P->orphan_words
+                                                          : (size_t)P->C * P->orphan_words;
     bool ok = hipMemcpyAsync(P->d_rp, P->h_rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
                              P->stream) == hipSuccess;
     if (P->nnz)
         ok = ok && hipMemcpyAsync(P->d_ci, ci.data(), sizeof(int32_t) * P->nnz, hipMemcpyHostToDevice,
                                   P->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * (size_t)P->C * P->orphan_words,
-                              P->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * mark_words, P->stream) == hipSuccess;
+    ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * orph_words, P->stream) == hipSuccess;
+    std::vector<uint32_t> bst;
+    if (P->mode == GSOFA_MODE_BLOCK) {
+        ok = ok && hipMemsetAsync(P->d_lab, 0xFF, sizeof(uint32_t) * P->lab_words, P->stream) == hipSuccess;
+        bst.resize(2 * (size_t)P->nblocks);
+        for (uint32_t b = 0; b < P->nblocks; b++) {
+            bst[2 * b] = 0xFFFFFFFFu - (uint32_t)std::max(n, 1);
+            bst[2 * b + 1] = 0;
+        }
+        ok = ok && hipMemcpyAsync(P->d_bstate, bst.data(), sizeof(uint32_t) * bst.size(),
+                                  hipMemcpyHostToDevice, P->stream) == hipSuccess;
+    }
     ok = ok && hipStreamSynchronize(P->stream) == hipSuccess;
     if (!ok) {
         (void)hipGetLastError();
@@ -459,22 +564,22 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
     P->hw = 0;
     P->epoch = 0;
 
-    // persistent grid: as many blocks as can be co-resident
-    int occ = 0;
-    hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, P->device) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, traverse_kernel_ptr(), traverse_block_size(),
-                                                     0) != hipSuccess ||
-        occ <= 0) {
-        (void)hipGetLastError();
-        release_device(P);
-        (void)hipStreamDestroy(P->stream);
-        delete P;
-        return GSOFA_ERR_HIP;
+    if (P->mode == GSOFA_MODE_GRID) {
+        // persistent grid: as many blocks as can be co-resident
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, traverse_kernel_ptr(), traverse_block_size(),
+                                                         0) != hipSuccess ||
+            occ <= 0) {
+            (void)hipGetLastError();
+            release_device(P);
+            (void)hipStreamDestroy(P->stream);
+            delete P;
+            return GSOFA_ERR_HIP;
+        }
+        uint32_t maxblocks = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
+        P->nblocks = P->opt.grid_blocks > 0 ? std::min<uint32_t>((uint32_t)P->opt.grid_blocks, maxblocks)
+                                             : std::min<uint32_t>(maxblocks, (uint32_t)prop.multiProcessorCount);
     }
-    uint32_t maxblocks = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
-    P->nblocks = P->opt.grid_blocks > 0 ? std::min<uint32_t>((uint32_t)P->opt.grid_blocks, maxblocks)
-                                         : maxblocks;
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, P->device) != hipSuccess ||
         rate_khz <= 0) {
@@ -529,6 +634,34 @@ int gsofa_plan_execute(gsofa_plan* P) {
                 }
             }
             P->marks_clean = false;
+            if (P->mode == GSOFA_MODE_BLOCK) {
+                BlkParams bp{};
+                bp.rp = P->d_rp;
+                bp.ci = P->d_ci;
+                bp.n = n;
+                bp.n_gpu = N;
+                bp.gpu_id = gid;
+                bp.slots = P->S;
+                bp.nblocks = P->nblocks;
+                bp.block_threshold = P->opt.wave_block_threshold;
+                bp.lab = P->d_lab;
+                bp.mark = P->d_mark;
+                bp.orphan = P->d_orphan;
+                bp.ow = P->orphan_words;
+                bp.ql_cap = P->ql_cap;
+                bp.qg_cap = P->qg_cap;
+                bp.stamp_base = P->epoch * (uint32_t)n;
+                bp.qg = P->d_qg;
+                bp.fills = P->d_fills;
+                bp.fill_cap = P->fill_cap;
+                bp.ctrl = P->d_ctrl;
+                bp.bstate = P->d_bstate;
+                HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
+                HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+                if (cnt > 0) HIPCHK(launch_traverse_blk(bp, P->blk_lds, P->stream));
+                HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+                continue;
+            }
             TravParams tp{};
             tp.rp = P->d_rp;
             tp.ci = P->d_ci;
@@ -660,8 +793,9 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->share_ms_min = P->share_min;
     r->share_ms_max = P->share_max;
     r->chunk = P->C;
+    r->mode = P->mode;
     r->grid_blocks = (int32_t)P->nblocks;
-    r->queue_capacity = P->Q;
+    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->ql_cap + P->qg_cap : (int64_t)P->Q;
     r->n_super = -1;
     if (!P->structure || n == 0) {
         if (n == 0) {

@@ -25,7 +25,42 @@ struct Ctrl {
     unsigned long long edge_checks;
     unsigned long long levels;
     unsigned long long rescans;
+    uint32_t src_next;            // next share-local source index (block mode)
+    uint32_t pad1;
+    // decision inputs published by the last block to arrive at a grid barrier
+    // (slot gen & 1): queue size of the next phase and outstanding orphans
+    unsigned long long snap_q[2];
+    unsigned long long snap_orph[2];
 };
+
+constexpr int kMaxSlots = 128;    // slots per workgroup in block mode (7-bit field)
+
+// Block-resident mode: each workgroup owns `slots` source slots.
+struct BlkParams {
+    const int32_t* rp;
+    const int32_t* ci;
+    int32_t n;
+    int32_t n_gpu;
+    int32_t gpu_id;
+    uint32_t slots;        // source slots per workgroup
+    uint32_t nblocks;
+    int32_t block_threshold;
+    uint32_t* lab;         // nblocks * slots * n labels
+    uint32_t* mark;        // nblocks * slots * n markers
+    uint32_t* orphan;      // nblocks * slots * ow overflow bits
+    uint32_t ow;
+    uint32_t ql_cap;       // LDS entries per queue buffer
+    uint32_t qg_cap;       // global entries per queue buffer per workgroup
+    uint32_t stamp_base;
+    uint64_t* qg;          // nblocks * 2 * qg_cap
+    uint64_t* fills;
+    uint64_t fill_cap;
+    Ctrl* ctrl;
+    uint32_t* bstate;      // per workgroup: max-id offset, groups since re-init
+};
+int blk_block_size();
+const void* blk_kernel_ptr();
+hipError_t launch_traverse_blk(const BlkParams& p, size_t lds_bytes, hipStream_t s);
 
 struct TravParams {
     const int32_t* rp;     // n+1 packed row pointers

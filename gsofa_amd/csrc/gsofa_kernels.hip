@@ -16,13 +16,15 @@
 // ballot + mbcnt into per-wave LDS staging buffers and flushed with one global
 // atomic per buffer; rows at least block_threshold long are expanded by the
 // whole workgroup at the end of the level.
+#include "gsofa_device.h"
 #include "gsofa_internal.h"
 
 namespace gsofa {
 
 namespace {
 
-constexpr int WAVE = 64;
+using namespace dev;
+
 constexpr int BLOCK = 256;
 constexpr int WAVES = BLOCK / WAVE;
 constexpr int PBUF = 256;   // staged frontier pairs per wave
@@ -43,87 +45,41 @@ struct BigRow {
     uint32_t cost, slot;
 };
 
+struct Snap {
+    unsigned long long q, orph;
+};
+
 struct BlockLds {
     WaveLds w[WAVES];
     BigRow big[BIGCAP];
     uint32_t nbig;
     uint32_t flag;
+    Snap snap;
 };
 
-__device__ __forceinline__ uint32_t rank_in(uint64_t mask) {
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-__device__ __forceinline__ void wave_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__device__ __forceinline__ uint32_t bcast0(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-__device__ __forceinline__ uint64_t bcast0_64(uint64_t v) {
-    uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-template <class T>
-__device__ __forceinline__ T ld_agent(T* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <class T>
-__device__ __forceinline__ void st_agent(T* p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// cached (L1-allowed) relaxed load: may return an older value, which is always
-// >= the current one for labels and <= for markers; used only as a filter.
-__device__ __forceinline__ uint32_t ld_cached(uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-__device__ __forceinline__ uint32_t amin(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint32_t amax(uint32_t* p, uint32_t v) {
-    return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long aadd64(unsigned long long* p, unsigned long long v) {
-    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Inclusive wave64 prefix sum.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
-    const uint32_t lane = __lane_id();
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, WAVE);
-        if (lane >= (uint32_t)d) x += y;
-    }
-    return x;
-}
-
-__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, WAVE);
-    return x;
-}
-
 // Atomic generation barrier over a co-resident (cooperatively launched) grid.
-// The last block to arrive resets the arrival count and bumps the generation;
-// the others poll the generation with acquire loads.  A wall-clock bound turns
-// a lost block into an abort flag instead of a hang.  Returns false on abort.
-__device__ bool grid_sync(Ctrl* c, uint32_t nblocks, uint64_t spin_ticks, uint32_t* s_flag) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
+// The last block to arrive resets the arrival count, snapshots the decision
+// inputs of the next phase (*snap_q and the orphan count) into the slot of this
+// generation, and bumps the generation; the others poll the generation with
+// acquire loads.  Every block then reads the same snapshot, so control flow
+// stays uniform even though the counters change again during the next phase.
+// A wall-clock bound turns a lost block into an abort flag instead of a hang.
+// Returns false on abort.
+__device__ bool grid_sync(Ctrl* c, uint32_t nblocks, uint64_t spin_ticks, uint32_t* s_flag,
+                          unsigned long long* snap_q, Snap* s_snap) {
+    __syncthreads();  // orders this block's writes before thread 0's release
     if (threadIdx.x == 0) {
-        uint32_t gen = ld_agent(&c->bar_gen);
-        uint32_t arrived =
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        const uint32_t gen = ld_agent(&c->bar_gen);
+        const uint32_t arrived =
             __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
             st_agent(&c->bar_count, 0u);
+            st_agent(&c->snap_q[gen & 1u], snap_q ? ld_agent(snap_q) : 0ull);
+            st_agent(&c->snap_orph[gen & 1u], ld_agent(&c->orphans));
             __hip_atomic_store(&c->bar_gen, gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            uint64_t t0 = wall_clock64();
+            const uint64_t t0 = wall_clock64();
             uint32_t polls = 0;
             while (__hip_atomic_load(&c->bar_gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
                 __builtin_amdgcn_s_sleep(2);
@@ -135,10 +91,12 @@ __device__ bool grid_sync(Ctrl* c, uint32_t nblocks, uint64_t spin_ticks, uint32
                 }
             }
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        s_snap->q = ld_agent(&c->snap_q[gen & 1u]);
+        s_snap->orph = ld_agent(&c->snap_orph[gen & 1u]);
         *s_flag = ld_agent(&c->abort);
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return *s_flag == 0;
 }
 
@@ -446,7 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TravParams p) {
                 st_agent(&c->cursor[j], 0u);
             }
         }
-        if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag)) return;
+        if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag, nullptr, &S.snap)) return;
 
         uint32_t pcount = 0, fcount = 0;
         Level lv0{p.qbuf0, &c->qcnt[0], p.qcap};
@@ -454,26 +412,28 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TravParams p) {
         if (pcount) flush_pushes(p, W, pcount, lv0);
         if (fcount) flush_fills(p, W, fcount);
         pcount = fcount = 0;
-        if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag)) return;
+        if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag, &c->qcnt[0], &S.snap)) return;
 
         // Level loop.  Level L reads queue L&1 with counter qcnt[L%3] and
         // cursor[L%3], and appends to queue (L+1)&1 with counter qcnt[(L+1)%3].
         // During level L block 0 clears the counters level L+1 will append
         // to / fetch from; the barrier that ends level L publishes them.
+        // All decisions use the snapshot taken at the preceding barrier.
         uint32_t L = 0;
         for (;;) {
             uint64_t* qc = (L & 1u) ? p.qbuf1 : p.qbuf0;
-            const unsigned long long appended = ld_agent(&c->qcnt[L % 3u]);
+            const unsigned long long appended = S.snap.q;
+            const unsigned long long orph = S.snap.orph;
             const uint32_t size = appended < p.qcap ? (uint32_t)appended : p.qcap;
-            if (size == 0) {
-                if (ld_agent(&c->orphans) == 0ull) break;
-                // drain-and-resume: refill this level's queue from the
+            if (size < p.qcap && orph != 0ull) {
+                // drain-and-resume: top this level's queue up from the
                 // overflow bitmap, then run the level as usual
                 rescan_orphans(p, g, qc, &c->qcnt[L % 3u]);
                 rescans++;
-                if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag)) return;
+                if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag, &c->qcnt[L % 3u], &S.snap)) return;
                 continue;
             }
+            if (size == 0) break;
             if (gtid == 0) {
                 st_agent(&c->qcnt[(L + 2u) % 3u], 0ull);
                 st_agent(&c->cursor[(L + 1u) % 3u], 0u);
@@ -493,7 +453,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TravParams p) {
             if (pcount) flush_pushes(p, W, pcount, lv);
             if (fcount) flush_fills(p, W, fcount);
             pcount = fcount = 0;
-            if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag)) return;
+            if (!grid_sync(c, p.nblocks, spin_ticks, &S.flag, &c->qcnt[(L + 1u) % 3u], &S.snap)) return;
             L++;
             levels++;
         }

@@ -82,7 +82,20 @@ typedef struct gsofa_options {
     int64_t output_capacity;      /* fill entries reserved; 0 = automatic (grown on demand) */
     int32_t flags;                /* GSOFA_FLAG_*                                            */
     int32_t grid_blocks;          /* persistent grid size; 0 = occupancy-bounded maximum    */
+    int32_t mode;                 /* GSOFA_MODE_*                                           */
 } gsofa_options;
+
+/* Execution modes of the traversal (identical results):
+ * BLOCK: every workgroup of a persistent grid owns a few source slots and runs
+ *        their frontier levels alone (LDS queues, workgroup barriers only);
+ *        chunk = total concurrent sources.
+ * GRID:  one group of chunk sources at a time; all workgroups expand each
+ *        frontier level together, separated by a grid-wide generation barrier
+ *        (the reference's persistent kernel, without_space_optimization/
+ *        symbfact.cuh:706-962). */
+#define GSOFA_MODE_AUTO 0
+#define GSOFA_MODE_BLOCK 1
+#define GSOFA_MODE_GRID 2
 
 void gsofa_options_default(gsofa_options* opt);
 
@@ -110,7 +123,8 @@ typedef struct gsofa_result {
     double traverse_ms;   /* device time of the traversal kernel(s) alone              */
     double share_ms_min;  /* fastest / slowest share (reference log, symbfact.cuh:2530) */
     double share_ms_max;
-    int32_t chunk;        /* concurrent sources per group actually used                */
+    int32_t chunk;        /* concurrent sources actually used                          */
+    int32_t mode;         /* GSOFA_MODE_BLOCK or GSOFA_MODE_GRID                       */
     int32_t grid_blocks;  /* persistent grid actually used                             */
     int64_t queue_capacity;
     uint64_t levels;      /* frontier levels summed over groups                        */

@@ -18,6 +18,7 @@ from gpu_util import assert_same_structure, record_of
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(ROOT, "tests", "golden", "goldens.json")
+MODES = ["block", "grid"]
 
 
 def _say(*a):
@@ -30,60 +31,71 @@ def oracle_of(A, **kw):
     return oracle.symbfact(A.colbeg, A.colend, A.csr, A.n, **kw)
 
 
-def test_bcsstk18_reference_total(bcsstk18):
+@pytest.mark.parametrize("mode", MODES)
+def test_bcsstk18_reference_total(bcsstk18, mode):
     import gsofa_amd as g
-    _say("case: bcsstk18 default options")
-    r = g.symbfact(bcsstk18)
+    _say("case: bcsstk18 mode", mode)
+    r = g.symbfact(bcsstk18, mode=mode)
+    assert r.mode == mode
     assert r.fill_count == 980136  # reference log LOG:2
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
-def test_grid64(bcsstk18):
+@pytest.mark.parametrize("mode", MODES)
+def test_grid64(mode):
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     A = cf.config1()
-    _say("case: grid64 natural")
-    r = g.symbfact(A)
+    _say("case: grid64 natural, mode", mode)
+    r = g.symbfact(A, mode=mode)
     assert r.fill_count == 500094
     assert r.reach == A.n * (A.n - 1) // 2
     assert_same_structure(r, oracle_of(A))
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("seed", range(3))
-def test_random_small(seed):
+def test_random_small(seed, mode):
     import gsofa_amd as g
     rng = np.random.default_rng(500 + seed)
     for trial in range(12):
         n = int(rng.integers(1, 300))
         A, _ = random_matrix(rng, n, float(rng.choice([0.005, 0.02, 0.08])),
                             sym=bool(trial % 4 == 0))
-        _say("case: random seed", seed, "trial", trial, "n", n)
-        r = g.symbfact(A, chunk=int(rng.choice([0, 1, 3, 16])))
+        _say("case: random seed", seed, "trial", trial, "n", n, "mode", mode)
+        r = g.symbfact(A, mode=mode, chunk=int(rng.choice([0, 1, 3, 16, 700])))
         assert_same_structure(r, oracle_of(A))
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("chunk", [1, 5, 64, 1000, 0])
-def test_chunk_invariance(bcsstk18, chunk):
+def test_chunk_invariance(bcsstk18, chunk, mode):
     import gsofa_amd as g
-    _say("case: bcsstk18 chunk", chunk)
-    r = g.symbfact(bcsstk18, chunk=chunk)
+    _say("case: bcsstk18 chunk", chunk, "mode", mode)
+    r = g.symbfact(bcsstk18, chunk=chunk, mode=mode)
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
-def test_forced_queue_overflow(bcsstk18):
-    """A tiny frontier queue forces the drain-and-resume path; the result must
-    not change (the reference's spill path keeps its count, LOG:157-165)."""
+@pytest.mark.parametrize("mode,qcap,chunk", [("grid", 4096, 256), ("block", 64, 0),
+                                             ("block", 1000, 0)])
+def test_forced_queue_overflow(bcsstk18, mode, qcap, chunk):
+    """A small frontier queue forces the drain-and-resume path; the result must
+    not change (the reference's spill path keeps its count, LOG:157-165).  The
+    grid-mode capacity is chosen so that the run needs a few thousand levels
+    (a CPU emulation of the level loop gives 3,206 levels + 1,366 refills);
+    capacity 64 at chunk 256 needs ~5e5 grid-wide barriers."""
     import gsofa_amd as g
-    _say("case: bcsstk18 queue 64")
-    r = g.symbfact(bcsstk18, queue_capacity=64, chunk=256)
+    _say("case: bcsstk18 mode", mode, "queue", qcap, "chunk", chunk)
+    r = g.symbfact(bcsstk18, queue_capacity=qcap, chunk=chunk, mode=mode)
     assert r.rescans > 0
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
-def test_block_expansion_path(bcsstk18):
+@pytest.mark.parametrize("mode", MODES)
+def test_block_expansion_path(bcsstk18, mode):
     import gsofa_amd as g
-    _say("case: bcsstk18 block threshold 8")
-    r = g.symbfact(bcsstk18, wave_block_threshold=8)
+    _say("case: bcsstk18 block threshold 8, mode", mode)
+    r = g.symbfact(bcsstk18, wave_block_threshold=8, mode=mode)
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
@@ -102,21 +114,22 @@ def test_small_grid_and_budget(bcsstk18):
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("shares", [3, 7])
-def test_virtual_gpu_shares(bcsstk18, shares):
+def test_virtual_gpu_shares(bcsstk18, shares, mode):
     import gsofa_amd as g
     full = oracle_of(bcsstk18)
     total = 0
     for gid in range(shares):
-        _say("case: bcsstk18 share", gid, "of", shares)
-        r = g.symbfact(bcsstk18, n_virtual_gpu=shares, virtual_gpu_id=gid, chunk=97)
+        _say("case: bcsstk18 share", gid, "of", shares, "mode", mode)
+        r = g.symbfact(bcsstk18, n_virtual_gpu=shares, virtual_gpu_id=gid, chunk=97, mode=mode)
         o = oracle_of(bcsstk18, n_virtual_gpu=shares, virtual_gpu_id=gid)
         assert_same_structure(r, o, check_super=False)
         assert r.n_super == -1
         total += r.fill_count
     assert total == full.fill_count
     # every share in turn on one plan (the reference's virtual-GPU loop)
-    r = g.symbfact(bcsstk18, n_virtual_gpu=shares)
+    r = g.symbfact(bcsstk18, n_virtual_gpu=shares, mode=mode)
     assert_same_structure(r, full)
 
 
@@ -128,11 +141,12 @@ def test_supernode_cap(bcsstk18, max_size):
     assert_same_structure(r, oracle_of(bcsstk18), max_size=max_size)
 
 
-def test_plan_repeated_execute(bcsstk18):
+@pytest.mark.parametrize("mode", MODES)
+def test_plan_repeated_execute(bcsstk18, mode):
     """Stamps and offsets carry across executions without clearing."""
     import gsofa_amd as g
     o = oracle_of(bcsstk18)
-    with g.Plan(bcsstk18, chunk=512) as p:
+    with g.Plan(bcsstk18, chunk=512, mode=mode) as p:
         for it in range(4):
             _say("case: plan execute", it)
             p.execute()
@@ -145,28 +159,31 @@ def test_counts_only(bcsstk18):
     assert r.fill_count == 980136 and r.L_rowptr is None
 
 
-def test_edge_cases():
+@pytest.mark.parametrize("mode", MODES)
+def test_edge_cases(mode):
     import gsofa_amd as g
     # single vertex; two vertices; a row without its diagonal; empty rows
     for rows in ([[0]], [[0, 1], [0, 1]], [[1], [0, 1]], [[0], [1], [2]],
                  [[0, 2], [1], [0, 2]], [[], [1, 0], [2, 1]]):
         A = g.CsrMatrix.from_rows(rows)
-        _say("case: edge rows", rows)
-        assert_same_structure(g.symbfact(A), oracle_of(A))
+        _say("case: edge rows", rows, "mode", mode)
+        assert_same_structure(g.symbfact(A, mode=mode), oracle_of(A))
 
 
-def test_grid3d_nd():
+@pytest.mark.parametrize("mode", MODES)
+def test_grid3d_nd(mode):
     import gsofa_amd as g
     A = g.gen_grid(20, 20, 20, nd_leaf=64, extra_prob=0.05, extra_dist=2, seed=9)
-    _say("case: 20^3 ND grid")
-    assert_same_structure(g.symbfact(A), oracle_of(A))
+    _say("case: 20^3 ND grid, mode", mode)
+    assert_same_structure(g.symbfact(A, mode=mode), oracle_of(A))
 
 
-def test_band_small():
+@pytest.mark.parametrize("mode", MODES)
+def test_band_small(mode):
     import gsofa_amd as g
     A = g.gen_band(6000, 256, 8.0, 3)
-    _say("case: band n=6000")
-    assert_same_structure(g.symbfact(A, chunk=1024), oracle_of(A))
+    _say("case: band n=6000, mode", mode)
+    assert_same_structure(g.symbfact(A, chunk=1024, mode=mode), oracle_of(A))
 
 
 def _golden(name):
@@ -178,13 +195,14 @@ def _golden(name):
     return d[name]
 
 
-def test_c2_full_golden():
+@pytest.mark.parametrize("mode", MODES)
+def test_c2_full_golden(mode):
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     gold = _golden("c2")
     A = cf.config2()
-    _say("case: C2 48^3 ND full size")
-    r = g.symbfact(A)
+    _say("case: C2 48^3 ND full size, mode", mode)
+    r = g.symbfact(A, mode=mode)
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
         assert rec[k] == gold[k], k
