@@ -164,6 +164,8 @@ class SymbfactResult:
                 return np.zeros(max(cnt, 0), dt) if ptr or cnt == 0 else None
             return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(cnt,)).copy()
         self.row_fill = arr(r.row_fill, n, C.c_int64, np.int64) if r.row_fill else None
+        self.row_te = arr(r.row_te, n, C.c_int64, np.int64) if r.row_te else None
+        self.row_reach = arr(r.row_reach, n, C.c_int64, np.int64) if r.row_reach else None
         self.L_rowptr = arr(r.L_rowptr, n + 1, C.c_int64, np.int64) if r.L_rowptr else None
         self.U_rowptr = arr(r.U_rowptr, n + 1, C.c_int64, np.int64) if r.U_rowptr else None
         self.L_col = (arr(r.L_col, int(self.L_rowptr[-1]), C.c_int32, np.int32)
@@ -197,15 +199,32 @@ def footprint(A, **kw):
 
 
 class Plan:
-    """A matrix uploaded once; execute() runs the device path, fetch() copies."""
+    """A matrix (or a batch of matrices) uploaded once; execute() runs the
+    device path, fetch() / fetch_matrix(i) copy results to the host."""
 
     def __init__(self, A, **kw):
         self._A = A  # keep the borrowed arrays alive during create
-        m = A.to_c()
         o = _options(**kw)
         h = C.c_void_p()
-        _check(_lib.lib().gsofa_plan_create(C.byref(m), C.byref(o), C.byref(h)), "plan_create")
+        if isinstance(A, (list, tuple)):
+            mats = (_lib.Matrix * len(A))(*[M.to_c() for M in A])
+            _check(_lib.lib().gsofa_batch_create(mats, len(A), C.byref(o), C.byref(h)), "batch_create")
+        else:
+            m = A.to_c()
+            _check(_lib.lib().gsofa_plan_create(C.byref(m), C.byref(o), C.byref(h)), "plan_create")
         self._h = h
+
+    @property
+    def matrix_count(self):
+        return int(_lib.lib().gsofa_plan_matrix_count(self._h))
+
+    def fetch_matrix(self, index):
+        r = _lib.Result()
+        _check(_lib.lib().gsofa_plan_fetch_matrix(self._h, index, C.byref(r)), "plan_fetch_matrix")
+        try:
+            return SymbfactResult(r)
+        finally:
+            _lib.lib().gsofa_result_free(C.byref(r))
 
     def execute(self):
         _check(_lib.lib().gsofa_plan_execute(self._h), "plan_execute")

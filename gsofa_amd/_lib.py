@@ -19,7 +19,7 @@ EXPORTS = [
     "gsofa_symbfact", "gsofa_plan_create", "gsofa_plan_execute", "gsofa_plan_fetch",
     "gsofa_plan_destroy", "gsofa_plan_footprint", "gsofa_read_ascii", "gsofa_write_ascii",
     "gsofa_gen_grid", "gsofa_gen_band", "gsofa_gen_side_loguniform", "gsofa_gen_uniform_int",
-    "gsofa_gen_unit",
+    "gsofa_gen_unit", "gsofa_batch_create", "gsofa_plan_matrix_count", "gsofa_plan_fetch_matrix",
 ]
 
 
@@ -50,7 +50,8 @@ class Result(C.Structure):
                 ("traverse_ms", C.c_double), ("share_ms_min", C.c_double),
                 ("share_ms_max", C.c_double), ("chunk", C.c_int32), ("mode", C.c_int32),
                 ("grid_blocks", C.c_int32),
-                ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64)]
+                ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64),
+                ("row_te", C.c_void_p), ("row_reach", C.c_void_p)]
 
 
 _lib = None
@@ -80,6 +81,12 @@ def lib():
     L.gsofa_plan_fetch.argtypes = [C.c_void_p, P(Result)]
     L.gsofa_plan_fetch.restype = C.c_int
     L.gsofa_plan_destroy.argtypes = [C.c_void_p]
+    L.gsofa_batch_create.argtypes = [P(Matrix), C.c_int32, P(Options), P(C.c_void_p)]
+    L.gsofa_batch_create.restype = C.c_int
+    L.gsofa_plan_matrix_count.argtypes = [C.c_void_p]
+    L.gsofa_plan_matrix_count.restype = C.c_int32
+    L.gsofa_plan_fetch_matrix.argtypes = [C.c_void_p, C.c_int32, P(Result)]
+    L.gsofa_plan_fetch_matrix.restype = C.c_int
     L.gsofa_plan_footprint.argtypes = [P(Matrix), P(Options)]
     L.gsofa_plan_footprint.restype = C.c_int64
     L.gsofa_read_ascii.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, P(Csr)]

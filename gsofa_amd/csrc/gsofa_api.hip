@@ -34,8 +34,19 @@ struct gsofa_plan {
     uint32_t S = 0;       // block mode: slots per workgroup
     uint32_t ql_cap = 0, qg_cap = 0;
     size_t blk_lds = 0;
-    uint64_t* d_qg = nullptr;
+    uint32_t* d_mask = nullptr;
+    uint32_t* d_qgv = nullptr;
     uint32_t* d_bstate = nullptr;
+    // matrices of a batch (block-diagonal global numbering) and the work table
+    int32_t n_mats = 1;
+    std::vector<uint32_t> h_mat_base;   // n_mats + 1
+    uint32_t* d_mat_base = nullptr;
+    uint32_t W = 0;                     // largest matrix
+    std::vector<BlkChunk> h_chunks;
+    std::vector<uint32_t> share_chunk_off;  // per share, into h_chunks
+    BlkChunk* d_chunks = nullptr;
+    unsigned long long* d_row_te = nullptr;
+    unsigned long long* d_row_reach = nullptr;
     uint32_t Q = 0;
     uint32_t nblocks = 0;
     int coop = 1;
@@ -83,6 +94,8 @@ struct gsofa_plan {
     bool supernodes = false;
     Ctrl stats{};
     double ms = 0, traverse_ms = 0, share_min = 0, share_max = 0;
+    gsofa_result host{};      // host copy of the last execute (per-matrix fetches)
+    bool host_valid = false;
     std::vector<hipEvent_t> ev;
     uint64_t spin_ticks = 0;
 };
@@ -251,8 +264,51 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
     }
 }
 
-// Block mode: LDS per workgroup for two workgroups per CU, slots per
-// workgroup from the option or 16, shrunk until the slot arrays fit.
+// Block mode: LDS for two workgroups per CU, as many workgroups as can be
+// co-resident (or as the chunk option asks: chunk = concurrent sources), shrunk
+// until the per-workgroup arrays (272 n bytes each) fit the budget.
+int64_t block_bytes(uint32_t nb, uint32_t W) {
+    return (int64_t)nb * (8ll * kBlockSlots + 16) * W + 8ll * nb;
+}
+
+// Work table: chunks of up to kBlockSlots sources of one matrix, in the order
+// they are handed out (decreasing global source).  Shares of a single matrix
+// get consecutive ranges of the table.
+void build_chunks(gsofa_plan* P) {
+    const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    P->h_chunks.clear();
+    P->share_chunk_off.assign((size_t)N + 1, 0);
+    if (P->n_mats == 1) {
+        for (int32_t gid = 0; gid < N; gid++) {
+            P->share_chunk_off[gid] = (uint32_t)P->h_chunks.size();
+            const uint64_t cnt = share_count(P->n, N, gid);
+            for (uint64_t c = 0; c * kBlockSlots < cnt; c++) {
+                const uint64_t k = cnt - 1 - c * kBlockSlots;
+                BlkChunk ch;
+                ch.first = (uint32_t)(gid + k * (uint64_t)N);
+                ch.count = (uint32_t)std::min<uint64_t>(kBlockSlots, cnt - c * kBlockSlots);
+                ch.stride = (uint32_t)N;
+                ch.mat = 0;
+                P->h_chunks.push_back(ch);
+            }
+        }
+        P->share_chunk_off[N] = (uint32_t)P->h_chunks.size();
+        return;
+    }
+    for (int32_t b = P->n_mats - 1; b >= 0; b--) {
+        const uint32_t base = P->h_mat_base[b], nb = P->h_mat_base[b + 1] - base;
+        for (uint32_t c = 0; c * kBlockSlots < nb; c++) {
+            BlkChunk ch;
+            ch.first = base + nb - 1 - c * kBlockSlots;
+            ch.count = std::min<uint32_t>(kBlockSlots, nb - c * kBlockSlots);
+            ch.stride = 1;
+            ch.mat = (uint32_t)b;
+            P->h_chunks.push_back(ch);
+        }
+    }
+    P->share_chunk_off[1] = (uint32_t)P->h_chunks.size();
+}
+
 int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
     const int32_t n = P->n;
     if (P->opt.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, P->opt.mem_budget_bytes);
@@ -266,18 +322,12 @@ int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
         max_blk = 65536;
     const int64_t per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
                                                                      : 65536;
-    int64_t target = std::min<int64_t>(max_blk, per_cu / 2);
-    int64_t dyn = target - (int64_t)fa.sharedSizeBytes - 512;
-    uint32_t ql = dyn > 16 * 64 ? (uint32_t)(dyn / 16) : 64;
-    uint32_t qg = 1u << 16;
-    if (P->opt.queue_capacity > 0) {
-        const uint64_t qc = (uint64_t)std::min<int64_t>(P->opt.queue_capacity, 1ll << 30);
-        ql = (uint32_t)std::min<uint64_t>(ql, qc);
-        qg = (uint32_t)(qc - ql);
-    }
+    const int64_t target = std::min<int64_t>(max_blk, per_cu / 2);
+    const int64_t dyn = target - (int64_t)fa.sharedSizeBytes - 512;
+    uint32_t ql = dyn > 8 * 64 ? (uint32_t)(dyn / 8) : 64;
+    if (P->opt.queue_capacity > 0) ql = (uint32_t)std::min<int64_t>(ql, std::max<int64_t>(P->opt.queue_capacity, 1));
     P->ql_cap = ql;
-    P->qg_cap = qg;
-    P->blk_lds = (size_t)ql * 16;
+    P->blk_lds = (size_t)ql * 8;
     if (P->blk_lds > 65536)
         (void)hipFuncSetAttribute(blk_kernel_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
                                   (int)P->blk_lds);
@@ -289,24 +339,18 @@ int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
     }
     uint32_t nb = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
     if (P->opt.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)P->opt.grid_blocks);
+    if (P->opt.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, P->opt.chunk / kBlockSlots));
     const int32_t N = std::max(1, P->opt.n_virtual_gpu);
     const uint64_t maxsrc = std::max<uint64_t>(share_count(n, N, 0), 1);
-    uint32_t S = P->opt.chunk > 0 ? (uint32_t)std::max<int64_t>(1, P->opt.chunk / (int64_t)nb) : 16u;
-    S = std::min<uint32_t>(S, (uint32_t)kMaxSlots);
-    // no more slots than sources
-    while (S > 1 && (uint64_t)nb * (S / 2) >= maxsrc) S /= 2;
-    if ((uint64_t)nb > maxsrc && S == 1) nb = (uint32_t)std::max<uint64_t>(maxsrc, 1);
-    const uint64_t ow = ((uint64_t)n + 31) / 32;
-    auto bytes = [&](uint32_t nbk, uint32_t s) {
-        return (int64_t)((uint64_t)nbk * s * (8ull * n + 4 * ow) + (uint64_t)nbk * 2 * qg * 8 + 8ull * nbk);
-    };
-    while (bytes(nb, S) > avail && S > 1) S /= 2;
-    while (bytes(nb, S) > avail && nb > 1) nb /= 2;
-    if (bytes(nb, S) > avail) return GSOFA_ERR_BUDGET;
+    const uint64_t nchunks = P->n_mats == 1 ? (maxsrc + kBlockSlots - 1) / kBlockSlots
+                                            : std::max<uint64_t>(P->h_chunks.size(), 1);
+    nb = (uint32_t)std::min<uint64_t>(nb, nchunks);
+    while (block_bytes(nb, P->W) > avail && nb > 1) nb = nb > 64 ? nb - nb / 8 : nb - 1;
+    if (block_bytes(nb, P->W) > avail) return GSOFA_ERR_BUDGET;
     P->nblocks = nb;
-    P->S = S;
-    P->C = (int32_t)(nb * S);
-    P->Q = ql + qg;
+    P->S = kBlockSlots;
+    P->C = (int32_t)(nb * kBlockSlots);
+    P->Q = ql;
     return GSOFA_OK;
 }
 
@@ -357,8 +401,13 @@ void release_device(gsofa_plan* P) {
     dfree(P->d_nsuper);
     dfree(P->d_sn64);
     dfree(P->d_snflag);
-    dfree(P->d_qg);
+    dfree(P->d_mask);
+    dfree(P->d_qgv);
     dfree(P->d_bstate);
+    dfree(P->d_mat_base);
+    dfree(P->d_chunks);
+    dfree(P->d_row_te);
+    dfree(P->d_row_reach);
 }
 
 int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
@@ -408,6 +457,8 @@ void gsofa_options_default(gsofa_options* o) {
 
 void gsofa_result_free(gsofa_result* r) {
     if (!r) return;
+    free(r->row_te);
+    free(r->row_reach);
     free(r->row_fill);
     free(r->L_rowptr);
     free(r->L_col);
@@ -428,7 +479,12 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     return sz.total_bytes;
 }
 
-int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_plan** out) {
+}  // extern "C"
+
+// A is the global matrix; for a batch it is block-diagonal with matrix b on
+// vertices [mat_base[b], mat_base[b+1]).
+static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int32_t n_mats,
+                            const gsofa_options* opt, gsofa_plan** out) {
     if (!out) return GSOFA_ERR_INVALID;
     *out = nullptr;
     int rc = validate(A);
@@ -436,6 +492,15 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
     gsofa_plan* P = new gsofa_plan();
     if (opt) P->opt = *opt;
     else gsofa_options_default(&P->opt);
+    P->n_mats = n_mats;
+    P->h_mat_base.assign(mat_base, mat_base + n_mats + 1);
+    P->W = 1;
+    for (int32_t b = 0; b < n_mats; b++) P->W = std::max<uint32_t>(P->W, mat_base[b + 1] - mat_base[b]);
+    if (n_mats > 1) {  // batches: one share, block mode
+        P->opt.n_virtual_gpu = 1;
+        P->opt.virtual_gpu_id = -1;
+        P->opt.mode = GSOFA_MODE_BLOCK;
+    }
     if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
     if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu) {
         delete P;
@@ -480,19 +545,24 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
     }
     int e = 0;
     if (P->mode == GSOFA_MODE_BLOCK) {
+        build_chunks(P);
         rc = size_block_mode(P, prop, std::max<int64_t>(avail, 1 << 20));
         if (rc) {
             (void)hipStreamDestroy(P->stream);
             delete P;
             return rc;
         }
-        const size_t slots = (size_t)P->nblocks * P->S;
-        e |= dalloc(&P->d_lab, slots * (size_t)n);
-        e |= dalloc(&P->d_mark, slots * (size_t)n);
-        e |= dalloc(&P->d_orphan, slots * P->orphan_words);
-        e |= dalloc(&P->d_qg, (size_t)P->nblocks * 2 * std::max<uint32_t>(P->qg_cap, 1));
+        const size_t slots = (size_t)P->nblocks * kBlockSlots;
+        e |= dalloc(&P->d_lab, slots * (size_t)P->W);
+        e |= dalloc(&P->d_mark, slots * (size_t)P->W);
+        e |= dalloc(&P->d_mask, (size_t)P->nblocks * 2 * (size_t)P->W);
+        e |= dalloc(&P->d_qgv, (size_t)P->nblocks * 2 * (size_t)P->W);
         e |= dalloc(&P->d_bstate, 2 * (size_t)P->nblocks);
-        P->lab_words = slots * (size_t)n;
+        e |= dalloc(&P->d_mat_base, (size_t)n_mats + 1);
+        e |= dalloc(&P->d_chunks, std::max<size_t>(P->h_chunks.size(), 1));
+        e |= dalloc(&P->d_row_te, (size_t)n);
+        e |= dalloc(&P->d_row_reach, (size_t)n);
+        P->lab_words = slots * (size_t)P->W;
     } else {
         Sizing sz;
         rc = choose_sizing(A, &P->opt, P->nnz, std::max<int64_t>(avail, 1 << 20), &sz);
@@ -529,27 +599,35 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
         delete P;
         return GSOFA_ERR_ALLOC;
     }
-    const size_t mark_words = P->mode == GSOFA_MODE_BLOCK ? (size_t)P->nblocks * P->S * n
+    const size_t mark_words = P->mode == GSOFA_MODE_BLOCK ? (size_t)P->nblocks * kBlockSlots * P->W
                                                           : (size_t)P->C * n;
-    const size_t orph_words = P->mode == GSOFA_MODE_BLOCK ? (size_t)P->nblocks * P->S * P->orphan_words
-                                                          : (size_t)P->C * P->orphan_words;
     bool ok = hipMemcpyAsync(P->d_rp, P->h_rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
                              P->stream) == hipSuccess;
     if (P->nnz)
         ok = ok && hipMemcpyAsync(P->d_ci, ci.data(), sizeof(int32_t) * P->nnz, hipMemcpyHostToDevice,
                                   P->stream) == hipSuccess;
     ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * mark_words, P->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * orph_words, P->stream) == hipSuccess;
+    if (P->mode == GSOFA_MODE_GRID)
+        ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * (size_t)P->C * P->orphan_words,
+                                  P->stream) == hipSuccess;
+    else
+        ok = ok && hipMemsetAsync(P->d_mask, 0, sizeof(uint32_t) * (size_t)P->nblocks * 2 * P->W,
+                                  P->stream) == hipSuccess;
     std::vector<uint32_t> bst;
     if (P->mode == GSOFA_MODE_BLOCK) {
         ok = ok && hipMemsetAsync(P->d_lab, 0xFF, sizeof(uint32_t) * P->lab_words, P->stream) == hipSuccess;
         bst.resize(2 * (size_t)P->nblocks);
         for (uint32_t b = 0; b < P->nblocks; b++) {
-            bst[2 * b] = 0xFFFFFFFFu - (uint32_t)std::max(n, 1);
+            bst[2 * b] = 0xFFFFFFFFu - P->W;
             bst[2 * b + 1] = 0;
         }
         ok = ok && hipMemcpyAsync(P->d_bstate, bst.data(), sizeof(uint32_t) * bst.size(),
                                   hipMemcpyHostToDevice, P->stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(P->d_mat_base, P->h_mat_base.data(), sizeof(uint32_t) * (n_mats + 1),
+                                  hipMemcpyHostToDevice, P->stream) == hipSuccess;
+        if (!P->h_chunks.empty())
+            ok = ok && hipMemcpyAsync(P->d_chunks, P->h_chunks.data(), sizeof(BlkChunk) * P->h_chunks.size(),
+                                      hipMemcpyHostToDevice, P->stream) == hipSuccess;
     }
     ok = ok && hipStreamSynchronize(P->stream) == hipSuccess;
     if (!ok) {
@@ -591,6 +669,47 @@ int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_pla
     return GSOFA_OK;
 }
 
+extern "C" {
+
+int gsofa_plan_create(const gsofa_matrix* A, const gsofa_options* opt, gsofa_plan** out) {
+    if (!A) return GSOFA_ERR_INVALID;
+    const uint32_t mb[2] = {0u, (uint32_t)std::max(A->n, 0)};
+    return plan_create_impl(A, mb, 1, opt, out);
+}
+
+int gsofa_batch_create(const gsofa_matrix* mats, int32_t count, const gsofa_options* opt,
+                       gsofa_plan** out) {
+    if (!out || count < 1 || !mats) return GSOFA_ERR_INVALID;
+    *out = nullptr;
+    int64_t n = 0, nnz = 0;
+    for (int32_t b = 0; b < count; b++) {
+        int rc = validate(&mats[b]);
+        if (rc) return rc;
+        n += mats[b].n;
+        nnz += mats[b].nnz;
+    }
+    if (n > kMaxN || nnz >= (int64_t)INT32_MAX) return GSOFA_ERR_TOO_LARGE;
+    // block-diagonal global matrix: matrix b's vertices are shifted by base_b
+    std::vector<uint32_t> mb((size_t)count + 1, 0);
+    std::vector<int32_t> cb((size_t)n), ce((size_t)n), cs((size_t)nnz);
+    int64_t v = 0, k = 0;
+    for (int32_t b = 0; b < count; b++) {
+        const gsofa_matrix& M = mats[b];
+        mb[b] = (uint32_t)v;
+        for (int32_t i = 0; i < M.n; i++) {
+            cb[v + i] = (int32_t)k;
+            for (int64_t e = M.colbeg[i]; e < M.colend[i]; e++) cs[k++] = M.csr[e] + (int32_t)v;
+            ce[v + i] = (int32_t)k;
+        }
+        v += M.n;
+    }
+    mb[count] = (uint32_t)v;
+    gsofa_matrix G = {(int32_t)n, nnz, cb.data(), ce.data(), cs.data()};
+    return plan_create_impl(&G, mb.data(), count, opt, out);
+}
+
+int32_t gsofa_plan_matrix_count(const gsofa_plan* P) { return P ? P->n_mats : 0; }
+
 int gsofa_plan_execute(gsofa_plan* P) {
     if (!P) return GSOFA_ERR_INVALID;
     HIPCHK(hipSetDevice(P->device));
@@ -604,6 +723,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
     P->structure = !(P->opt.flags & GSOFA_FLAG_COUNTS_ONLY);
     P->supernodes = P->structure && all_rows && n > 0;
     P->executed = false;
+    P->host_valid = false;
     memset(&P->stats, 0, sizeof(P->stats));
     if (n == 0) {
         P->executed = true;
@@ -639,26 +759,26 @@ int gsofa_plan_execute(gsofa_plan* P) {
                 bp.rp = P->d_rp;
                 bp.ci = P->d_ci;
                 bp.n = n;
-                bp.n_gpu = N;
-                bp.gpu_id = gid;
-                bp.slots = P->S;
+                bp.W = P->W;
+                bp.chunks = P->d_chunks + P->share_chunk_off[gid];
+                bp.n_chunks = P->share_chunk_off[gid + 1] - P->share_chunk_off[gid];
+                bp.mat_base = P->d_mat_base;
+                bp.row_te = P->d_row_te;
+                bp.row_reach = P->d_row_reach;
                 bp.nblocks = P->nblocks;
-                bp.block_threshold = P->opt.wave_block_threshold;
                 bp.lab = P->d_lab;
                 bp.mark = P->d_mark;
-                bp.orphan = P->d_orphan;
-                bp.ow = P->orphan_words;
+                bp.mask = P->d_mask;
+                bp.qg = P->d_qgv;
                 bp.ql_cap = P->ql_cap;
-                bp.qg_cap = P->qg_cap;
                 bp.stamp_base = P->epoch * (uint32_t)n;
-                bp.qg = P->d_qg;
                 bp.fills = P->d_fills;
                 bp.fill_cap = P->fill_cap;
                 bp.ctrl = P->d_ctrl;
                 bp.bstate = P->d_bstate;
                 HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
                 HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
-                if (cnt > 0) HIPCHK(launch_traverse_blk(bp, P->blk_lds, P->stream));
+                if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->blk_lds, P->stream));
                 HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
                 continue;
             }
@@ -795,7 +915,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->chunk = P->C;
     r->mode = P->mode;
     r->grid_blocks = (int32_t)P->nblocks;
-    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->ql_cap + P->qg_cap : (int64_t)P->Q;
+    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->ql_cap + P->n : (int64_t)P->Q;
     r->n_super = -1;
     if (!P->structure || n == 0) {
         if (n == 0) {
@@ -817,12 +937,28 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         return GSOFA_ERR_ALLOC;
     }
     HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+    if (P->mode == GSOFA_MODE_BLOCK) {
+        r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+        r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+        if (!r->row_te || !r->row_reach) {
+            gsofa_result_free(r);
+            return GSOFA_ERR_ALLOC;
+        }
+        HIPCHK(hipMemcpyAsync(r->row_te, P->d_row_te, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                              P->stream));
+    }
     HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
     HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
     if (P->nnzL)
         HIPCHK(hipMemcpyAsync(r->L_col, P->d_Lcol, sizeof(int32_t) * P->nnzL, hipMemcpyDeviceToHost, P->stream));
     if (P->nnzU)
         HIPCHK(hipMemcpyAsync(r->U_col, P->d_Ucol, sizeof(int32_t) * P->nnzU, hipMemcpyDeviceToHost, P->stream));
+    if (r->row_te) {
+        HIPCHK(hipStreamSynchronize(P->stream));
+        for (int32_t i = 0; i < n; i++)
+            if (r->row_fill[i] < 0) r->row_te[i] = r->row_reach[i] = -1;
+    }
     if (P->supernodes) {
         int32_t ns = 0;
         HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
@@ -842,11 +978,86 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     return GSOFA_OK;
 }
 
+int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
+    if (!P || !r || index < 0 || index >= P->n_mats) return GSOFA_ERR_INVALID;
+    if (!P->executed) return GSOFA_ERR_STATE;
+    if (!P->host_valid) {
+        gsofa_result_free(&P->host);
+        int rc = gsofa_plan_fetch(P, &P->host);
+        if (rc) return rc;
+        P->host_valid = true;
+    }
+    const gsofa_result& G = P->host;
+    memset(r, 0, sizeof(*r));
+    const int64_t b0 = P->h_mat_base[index], b1 = P->h_mat_base[index + 1];
+    const int32_t nb = (int32_t)(b1 - b0);
+    r->n = nb;
+    r->ms = G.ms;
+    r->traverse_ms = G.traverse_ms;
+    r->chunk = G.chunk;
+    r->mode = G.mode;
+    r->grid_blocks = G.grid_blocks;
+    r->queue_capacity = G.queue_capacity;
+    r->n_super = -1;
+    if (!G.L_rowptr || !G.row_fill) return GSOFA_OK;  // counts-only batch
+    r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
+    r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+    const int64_t l0 = G.L_rowptr[b0], l1 = G.L_rowptr[b1], u0 = G.U_rowptr[b0], u1 = G.U_rowptr[b1];
+    r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
+    r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
+    if (G.row_te) {
+        r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
+        r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
+    }
+    if (!r->row_fill || !r->L_rowptr || !r->U_rowptr || !r->L_col || !r->U_col ||
+        (G.row_te && (!r->row_te || !r->row_reach))) {
+        gsofa_result_free(r);
+        return GSOFA_ERR_ALLOC;
+    }
+    uint64_t fills = 0, te = 0, reach = 0;
+    for (int32_t i = 0; i < nb; i++) {
+        r->row_fill[i] = G.row_fill[b0 + i];
+        fills += (uint64_t)std::max<int64_t>(r->row_fill[i], 0);
+        if (G.row_te) {
+            r->row_te[i] = G.row_te[b0 + i];
+            r->row_reach[i] = G.row_reach[b0 + i];
+            te += (uint64_t)std::max<int64_t>(r->row_te[i], 0);
+            reach += (uint64_t)std::max<int64_t>(r->row_reach[i], 0);
+        }
+    }
+    for (int32_t i = 0; i <= nb; i++) {
+        r->L_rowptr[i] = G.L_rowptr[b0 + i] - l0;
+        r->U_rowptr[i] = G.U_rowptr[b0 + i] - u0;
+    }
+    for (int64_t k = l0; k < l1; k++) r->L_col[k - l0] = G.L_col[k] - (int32_t)b0;
+    for (int64_t k = u0; k < u1; k++) r->U_col[k - u0] = G.U_col[k] - (int32_t)b0;
+    r->fill_count = fills;
+    r->te_alg = te;
+    r->reach = reach;
+    if (G.n_super >= 0 && G.supno && nb > 0) {
+        // supernodes never cross matrices: (i-1) is never in L_i at a boundary
+        const int32_t s0 = G.supno[b0], s1 = G.supno[b1 - 1];
+        r->n_super = s1 - s0 + 1;
+        r->super_start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(r->n_super + 1));
+        r->supno = (int32_t*)malloc(sizeof(int32_t) * (size_t)nb);
+        if (!r->super_start || !r->supno) {
+            gsofa_result_free(r);
+            return GSOFA_ERR_ALLOC;
+        }
+        for (int32_t s = 0; s < r->n_super; s++) r->super_start[s] = G.super_start[s0 + s] - (int32_t)b0;
+        r->super_start[r->n_super] = nb;
+        for (int32_t i = 0; i < nb; i++) r->supno[i] = G.supno[b0 + i] - s0;
+    }
+    return GSOFA_OK;
+}
+
 void gsofa_plan_destroy(gsofa_plan* P) {
     if (!P) return;
     (void)hipSetDevice(P->device);
     (void)hipStreamSynchronize(P->stream);
     for (auto e : P->ev) (void)hipEventDestroy(e);
+    gsofa_result_free(&P->host);
     release_device(P);
     (void)hipStreamDestroy(P->stream);
     delete P;

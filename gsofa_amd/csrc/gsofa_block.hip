@@ -2,29 +2,43 @@
 //
 // Same per-source arithmetic as the reference's warp-centric level kernel
 // (with_space_optimization/symbfact.cuh:883-1082) and seeding (:639-721), with
-// a different execution model for CDNA4: each workgroup of a persistent,
-// occupancy-bounded grid owns S source slots and runs a "mini-group" of S
-// sources through all of its frontier levels by itself, synchronising only
-// with __syncthreads; the next mini-group is taken from one global counter.
-// There is no grid-wide barrier and no hot global counter on the level path.
+// an execution model built for CDNA4:
 //
-// * Sources are taken in decreasing order so that the deepest traversals start
-//   first and the tail of the run consists of short ones.  Marker stamps are
-//   stamp_base + (n - s), so they still increase per slot (no marker reset,
-//   with_space_optimization/symbfact.cuh:1006).
-// * Max-id labels are per slot with the no-reset offset of
-//   with_space_optimization/symbfact.cuh:2388-2407, kept per workgroup.
-// * A frontier entry carries the label it was relaxed with, so popping needs no
-//   fresh label read; when a (possibly stale, never too low) cached read of the
-//   label is already lower, a better entry for the same vertex was enqueued by
-//   whoever lowered it, and the entry is dropped.
-// * Frontier queues are two-tier: the first ql_cap entries of each level live
-//   in LDS, the rest in a per-workgroup global region; beyond that, entries are
-//   parked in a per-slot overflow bitmap and re-enqueued when the queue drains.
-// * Wave64 merge-path expansion: a wave takes 64 entries, scans their row
-//   lengths and streams the concatenated CSR words across its lanes.
+// * Each workgroup of a persistent, occupancy-bounded grid owns 32 source
+//   slots and runs a "mini-group" of 32 consecutive sources through all of
+//   their frontier levels by itself, synchronising only with __syncthreads.
+//   The next mini-group comes from one global counter.  There is no grid-wide
+//   barrier and no hot global counter on the level path.
+// * Labels and markers are stored slot-interleaved, [vertex][32 slots]: the
+//   32 sources' max-id labels of one vertex share one 128-byte line.  Nearby
+//   sources reach mostly the same vertices in the same levels, so the frontier
+//   is kept per vertex with a 32-bit mask of the slots that must expand it
+//   (deduplicated: a vertex enters the next queue when its mask leaves zero).
+// * A half-wave (32 lanes, lane = slot) expands one frontier vertex for all of
+//   its slots: the row is read once, and each CSR word costs one coalesced
+//   line access for the 32 labels.  Edge checks stay per source: lane j relaxes
+//   label(j, m) with its own cost max(v, label(j, v)) exactly as the reference
+//   does for source j.
+// * Sources are taken in decreasing order so the deepest traversals start
+//   first; marker stamps are stamp_base + (n - s), still increasing per slot,
+//   so markers are never reset (symbfact.cuh:1006).  Labels use the per-group
+//   decreasing offset of symbfact.cuh:2388-2407, kept per workgroup.
+// * The frontier queue is two-tier: the first ql_cap entries of a level in
+//   LDS, the rest in a per-workgroup global region of W entries; with the
+//   per-vertex deduplication it can never overflow.
+// * Work is a table of chunks (up to 32 sources of one matrix).  A batch of
+//   matrices is one block-diagonal matrix in which no traversal can leave its
+//   own matrix, so per-workgroup arrays are indexed relative to the chunk's
+//   matrix and sized by the largest matrix W, and labels hold
+//   offset + (vertex - matrix base).
 #include "gsofa_device.h"
 #include "gsofa_internal.h"
+
+// Scope of the label/marker filter loads.  Agent scope reads the L2 (where the
+// atomics land) rather than a possibly stale L1 line.
+#ifndef GSOFA_FILTER_SCOPE
+#define GSOFA_FILTER_SCOPE __HIP_MEMORY_SCOPE_AGENT
+#endif
 
 namespace gsofa {
 
@@ -32,72 +46,52 @@ namespace {
 
 using namespace dev;
 
-constexpr int BB = 512;           // threads per workgroup
-constexpr int BW = BB / WAVE;     // waves per workgroup
-constexpr int FB = 128;           // staged fills per wave
-constexpr int BIGCAP = 32;        // workgroup-expanded rows per level
-constexpr uint32_t VBITS = 25;    // vertex / relative-label field width
-constexpr uint32_t VMASK = (1u << VBITS) - 1u;
-
-struct BigRowB {
-    int32_t beg, end;
-    uint32_t cost, slot;
-};
+constexpr int BB = 512;          // threads per workgroup
+constexpr int BW = BB / WAVE;    // waves per workgroup
+constexpr int SL = kBlockSlots;  // slots per workgroup (lanes of a half-wave)
+constexpr int FB = 128;          // staged fills per wave
+#ifndef GSOFA_KB
+#define GSOFA_KB 8
+#endif
+constexpr int KB = GSOFA_KB;     // edges probed together per half-wave
 
 struct BlkLds {
-    uint32_t incl[BW][WAVE];
-    int32_t beg[BW][WAVE];
-    uint32_t cost[BW][WAVE];
-    uint32_t slot[BW][WAVE];
     uint64_t fbuf[BW][FB];
-    BigRowB big[BIGCAP];
-    uint32_t src[kMaxSlots];
+    unsigned long long s_te[SL];     // per-slot TE of the current mini-group
+    unsigned long long s_reach[SL];  // per-slot |R_<(s)|
+    uint32_t src[SL];
     uint32_t qcnt[2];
     uint32_t cursor;
-    uint32_t nbig;
-    uint32_t orph;
-    uint32_t base;
-};
-
-struct Blk {
-    const BlkParams* p;
-    uint32_t* lab0;
-    uint32_t* mark0;
-    uint32_t* orph0;
-    uint64_t* qg0;   // 2 x qg_cap
-    uint64_t* ql;    // 2 x ql_cap (LDS)
-    uint32_t qtot;
-    uint32_t off, stale;
+    uint32_t chunk;
 };
 
 struct TallyB {
     unsigned long long fills, te, reach, checks;
 };
 
-// entry: hi = slot << 25 | (label - off), lo = vertex
-__device__ __forceinline__ uint64_t make_entry(uint32_t slot, uint32_t lab_rel, uint32_t v) {
-    return ((uint64_t)((slot << VBITS) | lab_rel) << 32) | v;
+struct Blk {
+    const BlkParams* p;
+    uint32_t* lab0;   // [W][SL]
+    uint32_t* mark0;  // [W][SL]
+    uint32_t* mask0;  // [2][W]
+    uint32_t* qg0;    // [2][W]
+    uint32_t* ql;     // [2][ql_cap] (LDS)
+    uint32_t W;
+};
+
+__device__ __forceinline__ uint32_t ld_filter(uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, GSOFA_FILTER_SCOPE);
 }
 
-__device__ __forceinline__ uint64_t q_read(const Blk& B, uint32_t buf, uint32_t i) {
+__device__ __forceinline__ uint32_t q_read(const Blk& B, uint32_t buf, uint32_t i) {
     const uint32_t qc = B.p->ql_cap;
-    return i < qc ? B.ql[buf * qc + i] : B.qg0[(uint64_t)buf * B.p->qg_cap + (i - qc)];
+    return i < qc ? B.ql[buf * qc + i] : B.qg0[(uint64_t)buf * B.W + (i - qc)];
 }
 
-__device__ __forceinline__ void park(const Blk& B, BlkLds& S, uint64_t e) {
-    const uint32_t v = (uint32_t)e;
-    const uint32_t slot = (uint32_t)(e >> 32) >> VBITS;
-    const uint32_t bit = 1u << (v & 31u);
-    uint32_t* w = B.orph0 + (uint64_t)slot * B.p->ow + (v >> 5);
-    const uint32_t old = __hip_atomic_fetch_or(w, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (!(old & bit)) atomicAdd(&S.orph, 1u);
-}
-
-__device__ __forceinline__ void q_write(const Blk& B, BlkLds& S, uint32_t buf, uint32_t i, uint64_t e) {
+__device__ __forceinline__ void q_write(const Blk& B, uint32_t buf, uint32_t i, uint32_t v) {
     const uint32_t qc = B.p->ql_cap;
-    if (i < qc) B.ql[buf * qc + i] = e;
-    else if (i < B.qtot) B.qg0[(uint64_t)buf * B.p->qg_cap + (i - qc)] = e;
-    else park(B, S, e);
+    if (i < qc) B.ql[buf * qc + i] = v;
+    else B.qg0[(uint64_t)buf * B.W + (i - qc)] = v;
 }
 
 __device__ void flush_fills_w(const BlkParams& p, uint64_t* fb, uint32_t cnt) {
@@ -113,288 +107,250 @@ __device__ void flush_fills_w(const BlkParams& p, uint64_t* fb, uint32_t cnt) {
     wave_sync();
 }
 
-// Wave-uniform: enqueue lanes with push, stage lanes with fill.
-__device__ __forceinline__ void emit(const Blk& B, BlkLds& S, uint64_t* fb, uint32_t& fcount,
-                                     uint32_t nbuf, bool push, uint64_t entry, bool fill,
-                                     uint64_t fentry) {
-    const uint64_t pm = __ballot(push);
-    if (pm) {
-        uint32_t pos = 0;
-        if (__lane_id() == 0) pos = atomicAdd(&S.qcnt[nbuf], (uint32_t)__popcll(pm));
-        pos = bcast0(pos);
-        if (push) q_write(B, S, nbuf, pos + rank_in(pm), entry);
-    }
+__device__ __forceinline__ void stage_fill(const BlkParams& p, uint64_t* fb, uint32_t& fcount,
+                                           bool fill, uint64_t entry) {
     const uint64_t fm = __ballot(fill);
     if (fm) {
         const uint32_t c = (uint32_t)__popcll(fm);
         if (fcount + c > (uint32_t)FB) {
-            flush_fills_w(*B.p, fb, fcount);
+            flush_fills_w(p, fb, fcount);
             fcount = 0;
         }
-        if (fill) fb[fcount + rank_in(fm)] = fentry;
+        if (fill) fb[fcount + rank_in(fm)] = entry;
         fcount += c;
     }
 }
 
-// One edge check (with_space_optimization/symbfact.cuh:989-1066).
-// Returns bit0 = enqueue m with label `cost`, bit1 = new fill.
-__device__ __forceinline__ uint32_t relax_b(const Blk& B, uint32_t m, uint32_t slot, uint32_t cost,
-                                            uint32_t src, TallyB& t) {
-    const BlkParams& p = *B.p;
-    const uint32_t n = (uint32_t)p.n;
-    const uint32_t stamp = p.stamp_base + (n - src);
-    uint32_t* mark = B.mark0 + (uint64_t)slot * n;
-    if (m > src) {
-        if (ld_cached(mark + m) != stamp && amax(mark + m, stamp) < stamp) return 2u;
-        return 0u;
-    }
-    uint32_t* lp = B.lab0 + (uint64_t)slot * n + m;
-    if (ld_cached(lp) <= cost) return 0u;
-    const uint32_t old = amin(lp, cost);
-    if (old <= cost) return 0u;
-    if (old >= B.stale) {  // first time m is reached from this source
-        t.reach++;
-        t.te += (uint32_t)(p.rp[m + 1] - p.rp[m]);
-    }
-    uint32_t r = (m < src) ? 1u : 0u;
-    if (m + B.off > cost && amax(mark + m, stamp) < stamp) r |= 2u;
-    return r;
-}
-
-__device__ void expand_strided_b(const Blk& B, BlkLds& S, uint64_t* fb, uint32_t& fcount,
-                                 uint32_t nbuf, TallyB& t, int32_t beg, int32_t end,
-                                 uint32_t cost, uint32_t slot, uint32_t lane0, uint32_t stride) {
-    const uint32_t src = S.src[slot];
-    for (int64_t e0 = beg; e0 < end; e0 += stride) {
-        const int64_t e = e0 + lane0;
-        uint32_t r = 0, m = 0;
-        if (e < end) {
-            m = (uint32_t)B.p->ci[e];
-            t.checks++;
-            r = relax_b(B, m, slot, cost, src, t);
-            if (r & 2u) t.fills++;
-        }
-        emit(B, S, fb, fcount, nbuf, r & 1u, make_entry(slot, cost - B.off, m), (r & 2u) != 0,
-             ((uint64_t)src << 32) | m);
-    }
-}
-
-__device__ void expand_level_b(const Blk& B, BlkLds& S, uint64_t* fb, uint32_t& fcount,
-                               uint32_t cbuf, uint32_t size, TallyB& t) {
-    const BlkParams& p = *B.p;
-    const uint32_t lane = __lane_id();
-    const uint32_t w = threadIdx.x / WAVE;
-    const uint32_t nbuf = cbuf ^ 1u;
-    const uint32_t n = (uint32_t)p.n;
-    const uint32_t T = (uint32_t)p.block_threshold;
-    for (;;) {
-        uint32_t i0 = 0;
-        if (lane == 0) i0 = atomicAdd(&S.cursor, (uint32_t)WAVE);
-        i0 = bcast0(i0);
-        if (i0 >= size) break;
-        const uint32_t i = i0 + lane;
-        uint32_t deg = 0, cost = 0, slot = 0;
-        int32_t b = 0;
-        if (i < size) {
-            const uint64_t e = q_read(B, cbuf, i);
-            const uint32_t v = (uint32_t)e;
-            const uint32_t hi = (uint32_t)(e >> 32);
-            slot = hi >> VBITS;
-            const uint32_t lab = B.off + (hi & VMASK);
-            // a lower label means a better entry for v is already queued
-            if (ld_cached(B.lab0 + (uint64_t)slot * n + v) >= lab) {
-                cost = (v + B.off) > lab ? (v + B.off) : lab;  // symbfact.cuh:924
-                b = p.rp[v];
-                const int32_t en = p.rp[v + 1];
-                deg = (uint32_t)(en - b);
-                if (deg >= T) {
-                    const uint32_t k = atomicAdd(&S.nbig, 1u);
-                    if (k < (uint32_t)BIGCAP) {
-                        S.big[k] = BigRowB{b, en, cost, slot};
-                        deg = 0;
-                    }
-                }
-            }
-        }
-        const uint32_t incl = wave_incl_scan(deg);
-        const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-        S.incl[w][lane] = incl;
-        S.beg[w][lane] = b - (int32_t)(incl - deg);
-        S.cost[w][lane] = cost;
-        S.slot[w][lane] = slot;
-        wave_sync();
-        for (uint32_t e0 = 0; e0 < total; e0 += WAVE) {
-            const uint32_t e = e0 + lane;
-            uint32_t r = 0, m = 0, sl = 0, src = 0, cst = 0;
-            if (e < total) {
-                uint32_t lo = 0, hi = WAVE - 1;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (S.incl[w][mid] > e) hi = mid;
-                    else lo = mid + 1;
-                }
-                m = (uint32_t)p.ci[S.beg[w][lo] + (int32_t)e];
-                sl = S.slot[w][lo];
-                cst = S.cost[w][lo];
-                src = S.src[sl];
-                t.checks++;
-                r = relax_b(B, m, sl, cst, src, t);
-                if (r & 2u) t.fills++;
-            }
-            emit(B, S, fb, fcount, nbuf, r & 1u, make_entry(sl, cst - B.off, m), (r & 2u) != 0,
-                 ((uint64_t)src << 32) | m);
-        }
-        wave_sync();
-    }
-}
-
-// Move parked entries from the overflow bitmap into queue `buf` while it has
-// room (entries beyond the room stay parked).
-__device__ void rescan_b(const Blk& B, BlkLds& S, uint32_t buf, uint32_t nsrc) {
-    const BlkParams& p = *B.p;
-    const uint64_t words = (uint64_t)nsrc * p.ow;
-    uint32_t taken = 0;
-    for (uint64_t wd = threadIdx.x; wd < words; wd += BB) {
-        uint32_t bits = B.orph0[wd];
-        if (!bits) continue;
-        const uint32_t c = (uint32_t)__popc(bits);
-        const uint32_t pos = atomicAdd(&S.qcnt[buf], c);
-        const uint32_t fit = pos >= B.qtot ? 0u : ((B.qtot - pos) < c ? (B.qtot - pos) : c);
-        const uint32_t slot = (uint32_t)(wd / p.ow);
-        const uint32_t vb = (uint32_t)(wd % p.ow) * 32u;
-        for (uint32_t k = 0; k < fit; k++) {
-            const uint32_t bp = (uint32_t)__ffs(bits) - 1u;
-            bits &= bits - 1u;
-            const uint32_t v = vb + bp;
-            // re-enqueue with the vertex's current label
-            const uint32_t lab = ld_fresh(B.lab0 + (uint64_t)slot * p.n + v);
-            q_write(B, S, buf, pos + k, make_entry(slot, lab - B.off, v));
-        }
-        B.orph0[wd] = bits;
-        taken += fit;
-    }
-    if (taken) atomicSub(&S.orph, taken);
-}
-
 __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
     __shared__ BlkLds S;
-    extern __shared__ uint64_t qlds[];
+    extern __shared__ uint32_t qlds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / WAVE;
+    const uint32_t half = lane >> 5;  // which half-wave
+    const uint32_t j = lane & 31u;    // slot of this lane
+    const bool leader = j == 0;
     const uint32_t n = (uint32_t)p.n;
-    const uint32_t N = (uint32_t)p.n_gpu;
-    const uint32_t Sl = p.slots;
-    const uint64_t cnt = ((uint64_t)n - (uint64_t)p.gpu_id + N - 1) / N;
-    const uint32_t loops = 0xFFFFFFFFu / n;
+    const uint32_t W = p.W;
+    const uint32_t loops = 0xFFFFFFFFu / W;
     Blk B;
     B.p = &p;
-    B.lab0 = p.lab + (uint64_t)blockIdx.x * Sl * n;
-    B.mark0 = p.mark + (uint64_t)blockIdx.x * Sl * n;
-    B.orph0 = p.orphan + (uint64_t)blockIdx.x * Sl * p.ow;
-    B.qg0 = p.qg + (uint64_t)blockIdx.x * 2 * p.qg_cap;
+    B.W = W;
+    B.lab0 = p.lab + (uint64_t)blockIdx.x * SL * W;
+    B.mark0 = p.mark + (uint64_t)blockIdx.x * SL * W;
+    B.mask0 = p.mask + (uint64_t)blockIdx.x * 2 * W;
+    B.qg0 = p.qg + (uint64_t)blockIdx.x * 2 * W;
     B.ql = qlds;
-    B.qtot = p.ql_cap + p.qg_cap;
     uint32_t off = p.bstate[2 * blockIdx.x];
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
-    unsigned long long levels = 0, rescans = 0;
+    unsigned long long levels = 0, tot_te = 0, tot_reach = 0;
     uint64_t* fb = S.fbuf[w];
     uint32_t fcount = 0;
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
         S.cursor = 0;
-        S.nbig = 0;
-        S.orph = 0;
     }
     for (;;) {
-        if (tid == 0) S.base = atomicAdd(&p.ctrl->src_next, Sl);
+        if (tid == 0) S.chunk = atomicAdd(&p.ctrl->src_next, 1u);
         __syncthreads();
-        const uint32_t base = S.base;
-        if ((uint64_t)base >= cnt) break;
-        const uint32_t nsrc = (uint32_t)((cnt - base) < Sl ? (cnt - base) : Sl);
-        if (tid < nsrc) S.src[tid] = (uint32_t)(p.gpu_id + (cnt - 1 - (base + tid)) * N);
+        const uint32_t ck = S.chunk;
+        if (ck >= p.n_chunks) break;
+        const BlkChunk C = p.chunks[ck];
+        const uint32_t nsrc = C.count;
+        const uint32_t wb = p.mat_base[C.mat];  // window base: vertex wb is local 0
+        if (tid < (uint32_t)SL) {
+            S.src[tid] = tid < nsrc ? C.first - tid * C.stride : 0u;
+            S.s_te[tid] = 0;
+            S.s_reach[tid] = 0;
+        }
         if (since >= loops) {  // offsets exhausted: re-initialise this block's labels
-            for (uint64_t i = tid; i < (uint64_t)Sl * n; i += BB) B.lab0[i] = 0xFFFFFFFFu;
-            off = 0xFFFFFFFFu - n;
+            for (uint64_t i = tid; i < (uint64_t)SL * W; i += BB) B.lab0[i] = 0xFFFFFFFFu;
+            off = 0xFFFFFFFFu - W;
             since = 0;
         }
         __syncthreads();
-        B.off = off;
-        B.stale = off + n;
-        // seeding (with_space_optimization/symbfact.cuh:639-721), one wave per slot
-        for (uint32_t j = w; j < nsrc; j += BW) {
-            const uint32_t src = S.src[j];
-            const uint32_t stamp = p.stamp_base + (n - src);
-            uint32_t* lab = B.lab0 + (uint64_t)j * n;
-            uint32_t* mark = B.mark0 + (uint64_t)j * n;
-            const int32_t b = p.rp[src], e = p.rp[src + 1];
+        const uint32_t stale = off + W;
+        // this lane's source (local id); lanes of slots >= nsrc never get a mask bit
+        const uint32_t my_srcg = S.src[j];
+        const uint32_t my_src = my_srcg - wb;
+        const uint32_t my_stamp = p.stamp_base + (n - my_srcg);
+
+        // ---- seeding (symbfact.cuh:639-721): wave w seeds slots w, w+BW, ...
+        for (uint32_t k = w; k < nsrc; k += BW) {
+            const uint32_t srcg = S.src[k];
+            const uint32_t src = srcg - wb;
+            const uint32_t stamp = p.stamp_base + (n - srcg);
+            const int32_t b = p.rp[srcg], e = p.rp[srcg + 1];
             if (lane == 0) {
-                lab[src] = off;
-                t.te += (uint32_t)(e - b);
+                B.lab0[(uint64_t)src * SL + k] = off;
+                atomicAdd(&S.s_te[k], (unsigned long long)(e - b));
                 t.checks += (uint32_t)(e - b);
             }
-            for (int32_t t0 = b; t0 < e; t0 += WAVE) {
-                const int32_t i = t0 + (int32_t)lane;
-                bool push = false;
-                uint32_t m = 0;
-                if (i < e) {
-                    m = (uint32_t)p.ci[i];
-                    mark[m] = stamp;
-                    if (m < src) {
-                        const uint32_t old = lab[m];
-                        if (old >= B.stale) {
-                            t.reach++;
-                            t.te += (uint32_t)(p.rp[m + 1] - p.rp[m]);
-                        }
-                        lab[m] = off;
-                        push = true;
+            for (int32_t i = b + (int32_t)lane; i < e; i += WAVE) {
+                const uint32_t mg = (uint32_t)p.ci[i];
+                const uint32_t m = mg - wb;
+                B.mark0[(uint64_t)m * SL + k] = stamp;
+                if (m < src) {
+                    const uint32_t old = B.lab0[(uint64_t)m * SL + k];
+                    if (old >= stale) {
+                        atomicAdd(&S.s_reach[k], 1ull);
+                        atomicAdd(&S.s_te[k], (unsigned long long)(p.rp[mg + 1] - p.rp[mg]));
                     }
+                    B.lab0[(uint64_t)m * SL + k] = off;
+                    const uint32_t o = __hip_atomic_fetch_or(B.mask0 + m, 1u << k, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+                    if (o == 0u) q_write(B, 0, atomicAdd(&S.qcnt[0], 1u), m);
                 }
-                emit(B, S, fb, fcount, 0, push, make_entry(j, 0, m), false, 0);
             }
         }
         __syncthreads();
+
+        // ---- level loop: relaxation (symbfact.cuh:883-1082), lane = slot
         uint32_t cur = 0;
         for (;;) {
-            // decision inputs: read by every thread, then a barrier before any
-            // wave can change them (parking, refills)
-            const uint32_t appended = S.qcnt[cur];
-            const uint32_t orph = S.orph;
-            __syncthreads();
-            const uint32_t size = appended < B.qtot ? appended : B.qtot;
-            if (size < B.qtot && orph != 0) {
-                // drain-and-resume: top the queue up from the overflow bitmap
-                rescan_b(B, S, cur, nsrc);
-                rescans++;
-                __syncthreads();
-                continue;
-            }
+            const uint32_t size = S.qcnt[cur];
+            __syncthreads();  // every thread has the size before it can change
             if (size == 0) break;
-            expand_level_b(B, S, fb, fcount, cur, size, t);
-            __syncthreads();
-            const uint32_t nbig = S.nbig < (uint32_t)BIGCAP ? S.nbig : (uint32_t)BIGCAP;
-            for (uint32_t r = 0; r < nbig; r++) {
-                const BigRowB br = S.big[r];
-                expand_strided_b(B, S, fb, fcount, cur ^ 1u, t, br.beg, br.end, br.cost, br.slot,
-                                 tid, BB);
+            const uint32_t nbuf = cur ^ 1u;
+            uint32_t* maskc = B.mask0 + (uint64_t)cur * W;
+            uint32_t* maskn = B.mask0 + (uint64_t)nbuf * W;
+            // Software pipeline: the queue entry, row bounds, slot mask and
+            // label of the next vertex pair are fetched while the current
+            // pair's edges are probed.
+            uint32_t i0 = 0;
+            if (lane == 0) i0 = atomicAdd(&S.cursor, 2u);
+            i0 = bcast0(i0);
+            bool nhave = false;
+            uint32_t nv = 0, nM = 0, nlv = 0;
+            int32_t nrb = 0, nre = 0;
+            auto prefetch = [&](uint32_t base_i) {
+                const uint32_t qi = base_i + half;
+                nhave = base_i < size && qi < size;
+                nv = 0;
+                nrb = nre = 0;
+                nM = 0;
+                nlv = 0;
+                if (nhave) {
+                    nv = q_read(B, cur, qi);
+                    nrb = p.rp[wb + nv];
+                    nre = p.rp[wb + nv + 1];
+                    nlv = ld_agent(B.lab0 + (uint64_t)nv * SL + j);
+                    if (leader)
+                        nM = __hip_atomic_exchange(maskc + nv, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            };
+            if (i0 < size) prefetch(i0);
+            while (i0 < size) {
+                const bool have = nhave;
+                const uint32_t v = nv;
+                const int32_t rb = nrb, re = nre;
+                const uint32_t Mv = nM, lv = nlv;
+                if (lane == 0) i0 = atomicAdd(&S.cursor, 2u);
+                i0 = bcast0(i0);
+                if (i0 < size) prefetch(i0);
+                const uint32_t M = __shfl(Mv, 0, 32);
+                const bool active = have && ((M >> j) & 1u);
+                const uint32_t cost = (v + off) > lv ? (v + off) : lv;  // symbfact.cuh:924
+                const uint32_t deg = have ? (uint32_t)(re - rb) : 0u;
+                const uint32_t d0 = __shfl(deg, 0, WAVE), d1 = __shfl(deg, 32, WAVE);
+                const uint32_t dmax = d0 > d1 ? d0 : d1;
+                for (uint32_t c0 = 0; c0 < dmax; c0 += 32) {
+                    // each half loads up to 32 CSR words of its row at once
+                    uint32_t cm = 0;
+                    if (c0 + j < deg) cm = (uint32_t)p.ci[rb + (int32_t)(c0 + j)] - wb;
+                    const uint32_t lim = dmax - c0 < 32u ? dmax - c0 : 32u;
+                    for (uint32_t k0 = 0; k0 < lim; k0 += KB) {
+                        // KB edges at a time: all label/marker probes in flight
+                        // together, then the atomics, then the bookkeeping
+                        uint32_t mm[KB], val[KB], old[KB];
+                        uint32_t* adr[KB];
+                        bool go[KB], up[KB], need[KB];
+#pragma unroll
+                        for (int u = 0; u < KB; u++) {
+                            mm[u] = __shfl(cm, (int)((k0 + u) & 31u), 32);
+                            go[u] = active && (k0 + u) < lim && (c0 + k0 + u) < deg;
+                            up[u] = mm[u] > my_src;
+                            adr[u] = (up[u] ? B.mark0 : B.lab0) + (uint64_t)mm[u] * SL + j;
+                        }
+#pragma unroll
+                        for (int u = 0; u < KB; u++) val[u] = go[u] ? ld_filter(adr[u]) : 0u;
+#pragma unroll
+                        for (int u = 0; u < KB; u++)
+                            need[u] = go[u] && (up[u] ? val[u] != my_stamp : val[u] > cost);
+#pragma unroll
+                        for (int u = 0; u < KB; u++)
+                            old[u] = need[u] ? (up[u] ? amax(adr[u], my_stamp) : amin(adr[u], cost)) : 0u;
+                        uint32_t r[KB];  // bit0: relaxed below the source, bit1: new fill
+#pragma unroll
+                        for (int u = 0; u < KB; u++) {
+                            r[u] = 0;
+                            if (go[u]) t.checks++;
+                            if (!need[u]) continue;
+                            if (up[u]) {
+                                r[u] = old[u] < my_stamp ? 2u : 0u;
+                            } else if (old[u] > cost) {
+                                const uint32_t m = mm[u];
+                                if (old[u] >= stale) {  // first reach of m from this source
+                                    t.reach++;
+                                    t.te += (uint32_t)(p.rp[wb + m + 1] - p.rp[wb + m]);
+                                }
+                                r[u] = m < my_src ? 1u : 0u;
+                                if (m + off > cost &&
+                                    amax(B.mark0 + (uint64_t)m * SL + j, my_stamp) < my_stamp)
+                                    r[u] |= 2u;
+                            }
+                            if (r[u] & 2u) t.fills++;
+                        }
+                        // slots that must expand m next level: one atomic per
+                        // half and edge, all issued before any result is used
+                        uint32_t bits[KB], prev[KB];
+#pragma unroll
+                        for (int u = 0; u < KB; u++)
+                            bits[u] = (uint32_t)(__ballot(r[u] & 1u) >> (32u * half));
+#pragma unroll
+                        for (int u = 0; u < KB; u++)
+                            prev[u] = (leader && bits[u])
+                                          ? __hip_atomic_fetch_or(maskn + mm[u], bits[u], __ATOMIC_RELAXED,
+                                                                  __HIP_MEMORY_SCOPE_AGENT)
+                                          : 1u;
+#pragma unroll
+                        for (int u = 0; u < KB; u++)
+                            if (leader && bits[u] && prev[u] == 0u)
+                                q_write(B, nbuf, atomicAdd(&S.qcnt[nbuf], 1u), mm[u]);
+#pragma unroll
+                        for (int u = 0; u < KB; u++)
+                            stage_fill(p, fb, fcount, (r[u] & 2u) != 0,
+                                       ((uint64_t)my_srcg << 32) | (mm[u] + wb));
+                    }
+                }
             }
             __syncthreads();
             if (tid == 0) {
                 S.qcnt[cur] = 0;
                 S.cursor = 0;
-                S.nbig = 0;
             }
             __syncthreads();
-            cur ^= 1u;
+            cur = nbuf;
             levels++;
         }
-        off -= n;
+        // per-row TE and |R_<(s)|: lane j's tallies belong to slot j
+        if (t.te) atomicAdd(&S.s_te[j], t.te);
+        if (t.reach) atomicAdd(&S.s_reach[j], t.reach);
+        t.te = t.reach = 0;
+        __syncthreads();
+        if (tid < nsrc) {
+            p.row_te[S.src[tid]] = S.s_te[tid];
+            p.row_reach[S.src[tid]] = S.s_reach[tid];
+            tot_te += S.s_te[tid];
+            tot_reach += S.s_reach[tid];
+        }
+        off -= W;
         since++;
     }
     if (fcount) flush_fills_w(p, fb, fcount);
-    const unsigned long long f = wave_sum64(t.fills), te = wave_sum64(t.te),
-                             rc = wave_sum64(t.reach), ch = wave_sum64(t.checks);
+    const unsigned long long f = wave_sum64(t.fills), te = wave_sum64(tot_te),
+                             rc = wave_sum64(tot_reach), ch = wave_sum64(t.checks);
     if (lane == 0) {
         if (f) aadd64(&p.ctrl->fill_count, f);
         if (te) aadd64(&p.ctrl->te, te);
@@ -403,7 +359,6 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
     }
     if (tid == 0) {
         aadd64(&p.ctrl->levels, levels);
-        aadd64(&p.ctrl->rescans, rescans);
         p.bstate[2 * blockIdx.x] = off;
         p.bstate[2 * blockIdx.x + 1] = since;
     }

@@ -33,30 +33,41 @@ struct Ctrl {
     unsigned long long snap_orph[2];
 };
 
-constexpr int kMaxSlots = 128;    // slots per workgroup in block mode (7-bit field)
+constexpr int kBlockSlots = 32;   // source slots per workgroup in block mode
 
-// Block-resident mode: each workgroup owns `slots` source slots.
+// Block-resident mode: each workgroup owns kBlockSlots source slots.
+// Work comes as a table of chunks, processed in table order (decreasing
+// sources): chunk c covers sources first, first - stride, ... (count <= 32) of
+// matrix `mat`, whose vertices are [mat_base[mat], mat_base[mat+1]) in the
+// global numbering of a block-diagonal batch (a single matrix is mat 0).
+struct BlkChunk {
+    uint32_t first;
+    uint32_t count;
+    uint32_t stride;
+    uint32_t mat;
+};
+
 struct BlkParams {
     const int32_t* rp;
     const int32_t* ci;
-    int32_t n;
-    int32_t n_gpu;
-    int32_t gpu_id;
-    uint32_t slots;        // source slots per workgroup
+    int32_t n;             // total vertices (all matrices of the batch)
+    uint32_t W;            // per-workgroup window: largest matrix size
+    const BlkChunk* chunks;
+    uint32_t n_chunks;
+    const uint32_t* mat_base;  // n_mats + 1 vertex offsets
     uint32_t nblocks;
-    int32_t block_threshold;
-    uint32_t* lab;         // nblocks * slots * n labels
-    uint32_t* mark;        // nblocks * slots * n markers
-    uint32_t* orphan;      // nblocks * slots * ow overflow bits
-    uint32_t ow;
+    uint32_t* lab;         // nblocks * W * kBlockSlots labels, [vertex][slot]
+    uint32_t* mark;        // nblocks * W * kBlockSlots markers, [vertex][slot]
+    uint32_t* mask;        // nblocks * 2 * W frontier slot masks
+    uint32_t* qg;          // nblocks * 2 * W queue entries (global tier)
     uint32_t ql_cap;       // LDS entries per queue buffer
-    uint32_t qg_cap;       // global entries per queue buffer per workgroup
     uint32_t stamp_base;
-    uint64_t* qg;          // nblocks * 2 * qg_cap
     uint64_t* fills;
     uint64_t fill_cap;
     Ctrl* ctrl;
     uint32_t* bstate;      // per workgroup: max-id offset, groups since re-init
+    unsigned long long* row_te;     // n: TE of each processed source row
+    unsigned long long* row_reach;  // n: |R_<(s)| of each processed source row
 };
 int blk_block_size();
 const void* blk_kernel_ptr();

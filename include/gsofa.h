@@ -129,6 +129,8 @@ typedef struct gsofa_result {
     int64_t queue_capacity;
     uint64_t levels;      /* frontier levels summed over groups                        */
     uint64_t rescans;     /* queue-overflow drain-and-resume passes                    */
+    int64_t* row_te;      /* n: TE of each processed row (block mode; NULL otherwise)  */
+    int64_t* row_reach;   /* n: |R_<(s)| of each processed row (block mode)            */
 } gsofa_result;
 
 void gsofa_result_free(gsofa_result* r);
@@ -149,6 +151,19 @@ int gsofa_plan_fetch(gsofa_plan* plan, gsofa_result* out);
 void gsofa_plan_destroy(gsofa_plan* plan);
 /* Device bytes a plan for A would hold (CSR + workspace + output reserve). */
 int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt);
+
+/* ---- batches ---------------------------------------------------------------
+ * Many independent matrices in one plan: they are laid out block-diagonally
+ * (matrix b's vertices follow matrix b-1's), so one persistent launch works
+ * through a table of (matrix, source-chunk) items.  No traversal can leave its
+ * matrix, so every matrix's result equals its stand-alone result.  No
+ * counterpart in the reference, whose driver handles one matrix per run
+ * (with_space_optimization/symbfact.cuh:1553). */
+int gsofa_batch_create(const gsofa_matrix* mats, int32_t count, const gsofa_options* opt,
+                       gsofa_plan** plan);
+int32_t gsofa_plan_matrix_count(const gsofa_plan* plan);
+/* Result of matrix `index` of the last execute, in that matrix's own numbering. */
+int gsofa_plan_fetch_matrix(gsofa_plan* plan, int32_t index, gsofa_result* out);
 
 /* ---- inputs: reference file format and generators ---------------------------
  * ASCII triple (with_space_optimization/symbfact.cuh:1591-1640): the csr file

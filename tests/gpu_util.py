@@ -15,6 +15,9 @@ def assert_same_structure(res, orc, check_super=True, max_size=0):
     np.testing.assert_array_equal(res.L_col, orc.L_col)
     np.testing.assert_array_equal(res.U_col, orc.U_col)
     np.testing.assert_array_equal(res.row_fill, orc.row_fill)
+    if getattr(res, "row_te", None) is not None:  # block mode reports per-row TE and |R|
+        np.testing.assert_array_equal(res.row_te, orc.row_te)
+        np.testing.assert_array_equal(res.row_reach, orc.row_reach)
     if check_super:
         ss, sn = oracle.supernodes(orc.L_rowptr, orc.L_col, orc.U_rowptr, max_size)
         assert res.n_super == ss.size - 1

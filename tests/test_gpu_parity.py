@@ -76,18 +76,28 @@ def test_chunk_invariance(bcsstk18, chunk, mode):
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
-@pytest.mark.parametrize("mode,qcap,chunk", [("grid", 4096, 256), ("block", 64, 0),
-                                             ("block", 1000, 0)])
-def test_forced_queue_overflow(bcsstk18, mode, qcap, chunk):
-    """A small frontier queue forces the drain-and-resume path; the result must
-    not change (the reference's spill path keeps its count, LOG:157-165).  The
-    grid-mode capacity is chosen so that the run needs a few thousand levels
-    (a CPU emulation of the level loop gives 3,206 levels + 1,366 refills);
-    capacity 64 at chunk 256 needs ~5e5 grid-wide barriers."""
+@pytest.mark.parametrize("qcap,chunk", [(4096, 256), (20000, 1000)])
+def test_forced_queue_overflow(bcsstk18, qcap, chunk):
+    """Grid mode: a small frontier queue forces the drain-and-resume path; the
+    result must not change (the reference's spill path keeps its count,
+    LOG:157-165).  The capacity is chosen so that the run needs a few thousand
+    levels (a CPU emulation of the level loop gives 3,206 levels + 1,366 refills
+    at 4096/256); capacity 64 at chunk 256 would need ~5e5 grid-wide barriers."""
     import gsofa_amd as g
-    _say("case: bcsstk18 mode", mode, "queue", qcap, "chunk", chunk)
-    r = g.symbfact(bcsstk18, queue_capacity=qcap, chunk=chunk, mode=mode)
+    _say("case: bcsstk18 grid mode, queue", qcap, "chunk", chunk)
+    r = g.symbfact(bcsstk18, queue_capacity=qcap, chunk=chunk, mode="grid")
     assert r.rescans > 0
+    assert_same_structure(r, oracle_of(bcsstk18))
+
+
+@pytest.mark.parametrize("qcap", [1, 64, 1000])
+def test_block_queue_spills_to_global(bcsstk18, qcap):
+    """Block mode: a small LDS tier pushes most frontier entries into the
+    per-workgroup global tier (deduplicated queues cannot overflow)."""
+    import gsofa_amd as g
+    _say("case: bcsstk18 block mode, LDS tier", qcap)
+    r = g.symbfact(bcsstk18, queue_capacity=qcap, mode="block")
+    assert r.queue_capacity == qcap + bcsstk18.n
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
@@ -209,3 +219,73 @@ def test_c2_full_golden(mode):
     from digests import array_digest
     assert r.n_super == gold["n_super0"]
     assert array_digest(r.super_start) == gold["super0"]
+
+
+def test_c3_full_golden():
+    """n=659,033 banded matrix: totals, L/U digest and supernodes vs the oracle."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    gold = _golden("c3")
+    A = cf.config3()
+    _say("case: C3 band n=659,033 full size, block mode")
+    r = g.symbfact(A, mode="block")
+    rec = record_of(r)
+    for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
+        assert rec[k] == gold[k], k
+    from digests import array_digest
+    assert r.n_super == gold["n_super0"]
+    assert array_digest(r.super_start) == gold["super0"]
+    # fill stays inside the +-256 band
+    assert int(np.max(np.diff(r.U_rowptr))) <= 256 and int(np.max(np.diff(r.L_rowptr))) <= 256
+
+
+def _check_gold(rec, gold, tag):
+    for k in ("fill_count", "te_alg", "reach", "lu", "row_fill"):
+        assert rec[k] == gold[k], (tag, k)
+
+
+def test_c4_one_by_one_goldens():
+    """The 256 matrices of the batch workload, one plan each, vs oracle goldens."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    golds = _golden("c4")
+    mats = cf.config4()
+    assert len(golds) == len(mats) == 256
+    for b in range(0, 256, 8):
+        _say("case: C4 matrix", b, "n", mats[b].n, "alone")
+        _check_gold(record_of(g.symbfact(mats[b])), golds[b], b)
+
+
+def test_c4_batch_one_launch():
+    """All 256 matrices in one plan / one persistent launch: every matrix's
+    result equals its golden (and so its stand-alone result)."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    from digests import array_digest
+    golds = _golden("c4")
+    mats = cf.config4()
+    _say("case: C4 batch of 256 in one plan")
+    with g.Plan(mats) as p:
+        assert p.matrix_count == 256
+        for it in range(2):
+            p.execute()
+            for b in range(256):
+                r = p.fetch_matrix(b)
+                _check_gold(record_of(r), golds[b], b)
+                assert r.n_super == golds[b]["n_super0"], b
+                assert array_digest(r.super_start) == golds[b]["super0"], b
+
+
+def test_random_batch_vs_oracle():
+    import gsofa_amd as g
+    rng = np.random.default_rng(77)
+    mats = []
+    for k in range(40):
+        n = int(rng.integers(1, 200))
+        mats.append(random_matrix(rng, n, float(rng.choice([0.01, 0.05, 0.15])))[0])
+    mats.append(g.gen_band(3000, 256, 8.0, 5))
+    _say("case: random batch of", len(mats))
+    with g.Plan(mats, max_supernode_size=4) as p:
+        p.execute()
+        for b, A in enumerate(mats):
+            assert_same_structure(p.fetch_matrix(b), oracle_of(A), max_size=4)
