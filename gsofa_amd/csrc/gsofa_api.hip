@@ -268,7 +268,7 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
 // co-resident (or as the chunk option asks: chunk = concurrent sources), shrunk
 // until the per-workgroup arrays (272 n bytes each) fit the budget.
 int64_t block_bytes(uint32_t nb, uint32_t W) {
-    return (int64_t)nb * (8ll * kBlockSlots + 16) * W + 8ll * nb;
+    return (int64_t)nb * ((8ll * kBlockSlots + 16) * W + 8);
 }
 
 // Work table: chunks of up to kBlockSlots sources of one matrix, in the order
@@ -322,18 +322,14 @@ int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
         max_blk = 65536;
     const int64_t per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
                                                                      : 65536;
-    const int64_t target = std::min<int64_t>(max_blk, per_cu / 2);
-    const int64_t dyn = target - (int64_t)fa.sharedSizeBytes - 512;
-    uint32_t ql = dyn > 8 * 64 ? (uint32_t)(dyn / 8) : 64;
-    if (P->opt.queue_capacity > 0) ql = (uint32_t)std::min<int64_t>(ql, std::max<int64_t>(P->opt.queue_capacity, 1));
-    P->ql_cap = ql;
-    P->blk_lds = (size_t)ql * 8;
-    if (P->blk_lds > 65536)
-        (void)hipFuncSetAttribute(blk_kernel_ptr(), hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)P->blk_lds);
+    (void)max_blk;
+    (void)per_cu;
+    P->ql_cap = 0;
+    P->blk_lds = 0;  // all LDS of the block kernel is static
     int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(), blk_block_size(),
-                                                     P->blk_lds) != hipSuccess || occ <= 0) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(), blk_block_size(), 0) !=
+            hipSuccess ||
+        occ <= 0) {
         (void)hipGetLastError();
         return GSOFA_ERR_HIP;
     }
@@ -350,7 +346,7 @@ int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
     P->nblocks = nb;
     P->S = kBlockSlots;
     P->C = (int32_t)(nb * kBlockSlots);
-    P->Q = ql;
+    P->Q = 0;
     return GSOFA_OK;
 }
 
@@ -770,7 +766,6 @@ int gsofa_plan_execute(gsofa_plan* P) {
                 bp.mark = P->d_mark;
                 bp.mask = P->d_mask;
                 bp.qg = P->d_qgv;
-                bp.ql_cap = P->ql_cap;
                 bp.stamp_base = P->epoch * (uint32_t)n;
                 bp.fills = P->d_fills;
                 bp.fill_cap = P->fill_cap;
@@ -915,7 +910,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->chunk = P->C;
     r->mode = P->mode;
     r->grid_blocks = (int32_t)P->nblocks;
-    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->ql_cap + P->n : (int64_t)P->Q;
+    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->W : (int64_t)P->Q;
     r->n_super = -1;
     if (!P->structure || n == 0) {
         if (n == 0) {

@@ -14,18 +14,29 @@
 //   sources reach mostly the same vertices in the same levels, so the frontier
 //   is kept per vertex with a 32-bit mask of the slots that must expand it
 //   (deduplicated: a vertex enters the next queue when its mask leaves zero).
-// * A half-wave (32 lanes, lane = slot) expands one frontier vertex for all of
-//   its slots: the row is read once, and each CSR word costs one coalesced
-//   line access for the 32 labels.  Edge checks stay per source: lane j relaxes
-//   label(j, m) with its own cost max(v, label(j, v)) exactly as the reference
-//   does for source j.
+// * A level runs in rounds of two phases, and no atomic touches global memory
+//   on the level path (on this part a global atomic executes at the memory
+//   side, not in the L2):
+//     A. combine: each half-wave pops a frontier vertex v, computes lane j's
+//        relaxation cost max(v, label_j(v)) for the slots j of v's mask
+//        (symbfact.cuh:924), looks each neighbour m up in an LDS hash table
+//        of the round's distinct targets, and folds the cost into that
+//        target's 32 per-slot cells with an LDS atomic min;
+//     B. relax: every table entry (a distinct target m) is relaxed once per
+//        slot against its label or marker exactly as the reference relaxes
+//        one edge (symbfact.cuh:989-1066), with the round's minimum cost.
+//        Targets are unique within a round, so plain loads and stores suffice.
+//   Folding several edges into one relaxation with their minimum cost gives
+//   the same labels, fills and first reaches as applying them one by one: the
+//   label ends at the minimum either way, and a fill is detected iff some
+//   cost is below the target (so iff the minimum is).
+//   A frontier vertex's own label may drop during a later round of the same
+//   level; that drop re-enqueues the vertex, so expanding it with the older
+//   cost is correct (label-correcting).
 // * Sources are taken in decreasing order so the deepest traversals start
 //   first; marker stamps are stamp_base + (n - s), still increasing per slot,
 //   so markers are never reset (symbfact.cuh:1006).  Labels use the per-group
 //   decreasing offset of symbfact.cuh:2388-2407, kept per workgroup.
-// * The frontier queue is two-tier: the first ql_cap entries of a level in
-//   LDS, the rest in a per-workgroup global region of W entries; with the
-//   per-vertex deduplication it can never overflow.
 // * Work is a table of chunks (up to 32 sources of one matrix).  A batch of
 //   matrices is one block-diagonal matrix in which no traversal can leave its
 //   own matrix, so per-workgroup arrays are indexed relative to the chunk's
@@ -34,67 +45,64 @@
 #include "gsofa_device.h"
 #include "gsofa_internal.h"
 
-// Scope of the label/marker filter loads.  Agent scope reads the L2 (where the
-// atomics land) rather than a possibly stale L1 line.
-#ifndef GSOFA_FILTER_SCOPE
-#define GSOFA_FILTER_SCOPE __HIP_MEMORY_SCOPE_AGENT
-#endif
-
 namespace gsofa {
 
 namespace {
 
 using namespace dev;
 
-constexpr int BB = 512;          // threads per workgroup
-constexpr int BW = BB / WAVE;    // waves per workgroup
-constexpr int SL = kBlockSlots;  // slots per workgroup (lanes of a half-wave)
-constexpr int FB = 128;          // staged fills per wave
+constexpr int BB = 512;              // threads per workgroup
+constexpr int BW = BB / WAVE;        // waves per workgroup
+constexpr int NH = 2 * BW;           // half-waves
+constexpr int SL = kBlockSlots;      // slots per workgroup (lanes of a half-wave)
+constexpr int T = kBlkTable;         // distinct targets per round
+constexpr int HB = 10;               // log2 of the hash slots
+constexpr int H = 1 << HB;           // hash slots (load factor <= T / H)
+constexpr int FB = WAVE;             // staged fills per wave (one ballot's worth)
+constexpr uint32_t INF = 0xFFFFFFFFu;
+constexpr uint32_t NONE = 0xFFFFFFFFu;   // no table entry (edge not in this group)
+constexpr uint32_t FULL = 0xFFFFFFFEu;   // table full: retry the edge next round
+constexpr uint32_t HOLE = 0xFFFFFFFFu;   // ent_m of an index that lost its insert
 #ifndef GSOFA_KB
-#define GSOFA_KB 8
+#define GSOFA_KB 4
 #endif
-constexpr int KB = GSOFA_KB;     // edges probed together per half-wave
+constexpr int KR = GSOFA_KB;         // table entries relaxed together per half-wave
+static_assert(T <= H / 2, "hash load factor");
 
 struct BlkLds {
-    uint64_t fbuf[BW][FB];
+    uint32_t cell[T][SL];            // min cost per (round target, slot); INF = none
+    unsigned long long hkey[H];      // ((m + 1) << 32) | table index; 0 = empty
+    uint32_t ent_m[T];               // table index -> target (window-local) or HOLE
+    uint32_t eidx[NH][32];           // table index of each lane's edge
+    uint32_t fbuf[BW][FB];           // staged fills, see flush_fills_w
     unsigned long long s_te[SL];     // per-slot TE of the current mini-group
     unsigned long long s_reach[SL];  // per-slot |R_<(s)|
     uint32_t src[SL];
     uint32_t qcnt[2];
     uint32_t cursor;
+    uint32_t tcnt;
+    uint32_t held;
     uint32_t chunk;
+};
+
+struct Blk {
+    const BlkParams* p;
+    uint32_t* lab0;    // [W][SL]
+    uint32_t* mark0;   // [W][SL]
+    uint32_t* mask0;   // [2][W]
+    uint32_t* q0;      // [2][W]
+    uint32_t W;
+    uint32_t wb;       // window base (global id of local vertex 0)
+    uint32_t off, stale;
 };
 
 struct TallyB {
     unsigned long long fills, te, reach, checks;
 };
 
-struct Blk {
-    const BlkParams* p;
-    uint32_t* lab0;   // [W][SL]
-    uint32_t* mark0;  // [W][SL]
-    uint32_t* mask0;  // [2][W]
-    uint32_t* qg0;    // [2][W]
-    uint32_t* ql;     // [2][ql_cap] (LDS)
-    uint32_t W;
-};
-
-__device__ __forceinline__ uint32_t ld_filter(uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, GSOFA_FILTER_SCOPE);
-}
-
-__device__ __forceinline__ uint32_t q_read(const Blk& B, uint32_t buf, uint32_t i) {
-    const uint32_t qc = B.p->ql_cap;
-    return i < qc ? B.ql[buf * qc + i] : B.qg0[(uint64_t)buf * B.W + (i - qc)];
-}
-
-__device__ __forceinline__ void q_write(const Blk& B, uint32_t buf, uint32_t i, uint32_t v) {
-    const uint32_t qc = B.p->ql_cap;
-    if (i < qc) B.ql[buf * qc + i] = v;
-    else B.qg0[(uint64_t)buf * B.W + (i - qc)] = v;
-}
-
-__device__ void flush_fills_w(const BlkParams& p, uint64_t* fb, uint32_t cnt) {
+// Staged fills are (slot << 25) | global target (n < 2^25); the flush turns
+// them into (source << 32) | target with the slot's source.
+__device__ void flush_fills_w(const BlkParams& p, const BlkLds& S, const uint32_t* fb, uint32_t cnt) {
     wave_sync();
     const uint32_t lane = __lane_id();
     uint64_t pos = 0;
@@ -102,18 +110,21 @@ __device__ void flush_fills_w(const BlkParams& p, uint64_t* fb, uint32_t cnt) {
     pos = bcast0_64(pos);
     for (uint32_t t = lane; t < cnt; t += WAVE) {
         const uint64_t at = pos + t;
-        if (at < p.fill_cap) p.fills[at] = fb[t];
+        const uint32_t f = fb[t];
+        if (at < p.fill_cap) p.fills[at] = ((uint64_t)S.src[f >> 25] << 32) | (f & 0x1FFFFFFu);
     }
     wave_sync();
 }
 
-__device__ __forceinline__ void stage_fill(const BlkParams& p, uint64_t* fb, uint32_t& fcount,
-                                           bool fill, uint64_t entry) {
+// Wave-uniform call: every lane of the wave must be active.  One call stages
+// at most 64 fills, so a 64-entry buffer never overflows.
+__device__ __forceinline__ void stage_fill(const BlkParams& p, const BlkLds& S, uint32_t* fb,
+                                           uint32_t& fcount, bool fill, uint32_t entry) {
     const uint64_t fm = __ballot(fill);
     if (fm) {
         const uint32_t c = (uint32_t)__popcll(fm);
         if (fcount + c > (uint32_t)FB) {
-            flush_fills_w(p, fb, fcount);
+            flush_fills_w(p, S, fb, fcount);
             fcount = 0;
         }
         if (fill) fb[fcount + rank_in(fm)] = entry;
@@ -121,15 +132,226 @@ __device__ __forceinline__ void stage_fill(const BlkParams& p, uint64_t* fb, uin
     }
 }
 
+// Table index of target m in this round, inserting it if new; FULL when the
+// table has no room left.  Linear probing over H slots; an index allocated for
+// an insert that loses its race to an equal key becomes a HOLE entry.
+__device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
+    const unsigned long long key = (unsigned long long)(m + 1u) << 32;
+    uint32_t s = (m * 2654435761u) >> (32 - HB);
+    uint32_t mine = NONE, res = FULL;
+    for (int probe = 0; probe < H; probe++) {
+        unsigned long long cur = S.hkey[s];
+        if (cur == 0ull) {
+            if (mine == NONE) {
+                mine = atomicAdd(&S.tcnt, 1u);
+                if (mine >= (uint32_t)T) {
+                    mine = NONE;
+                    break;
+                }
+            }
+            const unsigned long long prev = atomicCAS(&S.hkey[s], 0ull, key | mine);
+            if (prev == 0ull) {
+                S.ent_m[mine] = m;
+                res = mine;
+                mine = NONE;
+                break;
+            }
+            cur = prev;
+        }
+        if ((cur & 0xFFFFFFFF00000000ull) == key) {
+            res = (uint32_t)cur;
+            break;
+        }
+        s = (s + 1u) & (uint32_t)(H - 1);
+    }
+    if (mine != NONE) S.ent_m[mine] = HOLE;
+    return res;
+}
+
+// Frontier vertex held by a half-wave across rounds.  All lanes of the half
+// hold the same vertex; lane j holds slot j's cost.
+struct HalfVtx {
+    bool held;
+    uint32_t v, M, cost, deg, done, pend;
+    int32_t rb;
+};
+
+// Phase A: pop frontier vertices and fold their edges' costs into the table
+// until the queue is exhausted or the table is full.  Control flow diverges
+// between the two halves of a wave; nothing here needs the whole wave.
+__device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur, uint32_t size,
+                              TallyB& t) {
+    const BlkParams& p = *B.p;
+    const uint32_t lane = __lane_id();
+    const uint32_t half = lane >> 5;
+    const uint32_t j = lane & 31u;
+    const uint32_t h = (threadIdx.x / WAVE) * 2 + half;
+    uint32_t* maskc = B.mask0 + (uint64_t)cur * B.W;
+    const uint32_t* qc = B.q0 + (uint64_t)cur * B.W;
+    for (;;) {
+        if (!hv.held) {
+            if (S.tcnt >= (uint32_t)T) return;  // table full: take no new vertex
+            uint32_t i = 0;
+            if (j == 0) i = atomicAdd(&S.cursor, 1u);
+            i = __shfl(i, (int)(lane & 32u), WAVE);
+            if (i >= size) return;
+            hv.v = qc[i];
+            hv.M = maskc[hv.v];
+            if (j == 0) maskc[hv.v] = 0u;  // consumed; this buffer is refilled two levels later
+            hv.rb = p.rp[B.wb + hv.v];
+            hv.deg = (uint32_t)(p.rp[B.wb + hv.v + 1] - hv.rb);
+            hv.cost = INF;
+            if ((hv.M >> j) & 1u) {
+                const uint32_t lv = B.lab0[(uint64_t)hv.v * SL + j];
+                const uint32_t vo = hv.v + B.off;
+                hv.cost = vo > lv ? vo : lv;  // symbfact.cuh:924
+            }
+            if (j == 0) t.checks += (unsigned long long)hv.deg * (uint32_t)__popc(hv.M);
+            hv.done = 0;
+            hv.pend = hv.deg >= 32u ? 0xFFFFFFFFu : ((1u << hv.deg) - 1u);
+            hv.held = true;
+        }
+        // one group of up to 32 edges; pend = edges of the group not yet folded
+        const bool mine = (hv.pend >> j) & 1u;
+        uint32_t idx = NONE;
+        if (mine) idx = tab_find(S, (uint32_t)p.ci[hv.rb + (int32_t)(hv.done + j)] - B.wb);
+        S.eidx[h][j] = idx;
+        const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
+        wave_sync();
+        if (hv.cost != INF) {
+            for (uint32_t b = okm; b; b &= b - 1u) {
+                const uint32_t id = S.eidx[h][__builtin_ctz(b)];
+                atomicMin(&S.cell[id][j], hv.cost);
+            }
+        }
+        wave_sync();
+        hv.pend &= ~okm;
+        if (hv.pend) return;  // table full: the rest of the group waits
+        hv.done += 32u;
+        if (hv.done >= hv.deg) {
+            hv.held = false;
+        } else {
+            const uint32_t r = hv.deg - hv.done;
+            hv.pend = r >= 32u ? 0xFFFFFFFFu : ((1u << r) - 1u);
+        }
+    }
+}
+
+// Phase B: relax every table entry once per slot.  Wave-uniform trip count.
+__device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_src,
+                            uint32_t my_stamp, uint32_t* fb, uint32_t& fcount,
+                            TallyB& t) {
+    const BlkParams& p = *B.p;
+    const uint32_t lane = __lane_id();
+    const uint32_t half = lane >> 5;
+    const uint32_t j = lane & 31u;
+    const bool leader = j == 0;
+    const uint32_t w = threadIdx.x / WAVE;
+    uint32_t* maskn = B.mask0 + (uint64_t)nbuf * B.W;
+    uint32_t* qn = B.q0 + (uint64_t)nbuf * B.W;
+    const uint32_t tc = S.tcnt;
+    const uint32_t ntab = tc < (uint32_t)T ? tc : (uint32_t)T;
+    const uint32_t off = B.off;
+    for (uint32_t base = w * 2 * KR; base < ntab; base += BW * 2 * KR) {
+        uint32_t mm[KR], c[KR], lm[KR], nmk[KR];
+        bool ok[KR], act[KR], up[KR];
+#pragma unroll
+        for (int u = 0; u < KR; u++) {
+            const uint32_t e = base + half * KR + u;
+            ok[u] = e < ntab;
+            mm[u] = ok[u] ? S.ent_m[e] : HOLE;
+            ok[u] = ok[u] && mm[u] != HOLE;
+            c[u] = INF;
+            if (ok[u]) {
+                c[u] = S.cell[e][j];
+                S.cell[e][j] = INF;  // ready for the next round
+            }
+            act[u] = c[u] != INF;
+            up[u] = mm[u] > my_src;
+        }
+        // round trip 1: the target's label (or marker above the source) and
+        // its next-level slot mask
+#pragma unroll
+        for (int u = 0; u < KR; u++) {
+            lm[u] = act[u] ? (up[u] ? B.mark0 : B.lab0)[(uint64_t)mm[u] * SL + j] : 0u;
+            nmk[u] = (ok[u] && leader) ? maskn[mm[u]] : 0u;
+        }
+        bool first[KR], lcand[KR], push[KR], ufill[KR];
+#pragma unroll
+        for (int u = 0; u < KR; u++) {
+            first[u] = lcand[u] = push[u] = ufill[u] = false;
+            if (act[u]) {
+                if (up[u]) {  // m above the source: U entry (symbfact.cuh:992-1009)
+                    if (lm[u] != my_stamp) {
+                        B.mark0[(uint64_t)mm[u] * SL + j] = my_stamp;
+                        ufill[u] = true;
+                    }
+                } else if (lm[u] > c[u]) {  // relaxation (symbfact.cuh:1018-1063)
+                    B.lab0[(uint64_t)mm[u] * SL + j] = c[u];
+                    first[u] = lm[u] >= B.stale;
+                    push[u] = mm[u] < my_src;
+                    lcand[u] = mm[u] + off > c[u];
+                }
+            }
+        }
+        // round trip 2 (only when some lane needs it): L-fill markers and the
+        // degrees of first-reached vertices (TE_alg)
+        bool need2 = false;
+#pragma unroll
+        for (int u = 0; u < KR; u++) need2 = need2 || lcand[u] || first[u];
+        bool lfill[KR];
+#pragma unroll
+        for (int u = 0; u < KR; u++) lfill[u] = false;
+        if (__ballot(need2)) {
+            uint32_t mk[KR];
+            int32_t d0[KR], d1[KR];
+#pragma unroll
+            for (int u = 0; u < KR; u++) {
+                mk[u] = lcand[u] ? B.mark0[(uint64_t)mm[u] * SL + j] : 0u;
+                d0[u] = first[u] ? p.rp[B.wb + mm[u]] : 0;
+                d1[u] = first[u] ? p.rp[B.wb + mm[u] + 1] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < KR; u++) {
+                if (lcand[u] && mk[u] < my_stamp) {
+                    B.mark0[(uint64_t)mm[u] * SL + j] = my_stamp;
+                    lfill[u] = true;
+                }
+                if (first[u]) {
+                    t.reach++;
+                    t.te += (uint32_t)(d1[u] - d0[u]);
+                }
+            }
+        }
+        // next-level slot masks: the half leader writes maskn[m] (m is unique
+        // in the round, so no other lane touches it)
+        uint32_t bits[KR];
+#pragma unroll
+        for (int u = 0; u < KR; u++) bits[u] = (uint32_t)(__ballot(push[u]) >> (32u * half));
+        if (leader) {
+#pragma unroll
+            for (int u = 0; u < KR; u++) {
+                if (bits[u] && (nmk[u] | bits[u]) != nmk[u]) {
+                    maskn[mm[u]] = nmk[u] | bits[u];
+                    if (nmk[u] == 0u) qn[atomicAdd(&S.qcnt[nbuf], 1u)] = mm[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < KR; u++) {
+            const bool fill = ufill[u] || lfill[u];
+            if (fill) t.fills++;
+            stage_fill(p, S, fb, fcount, fill, (j << 25) | (mm[u] + B.wb));
+        }
+    }
+}
+
 __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
     __shared__ BlkLds S;
-    extern __shared__ uint32_t qlds[];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / WAVE;
-    const uint32_t half = lane >> 5;  // which half-wave
-    const uint32_t j = lane & 31u;    // slot of this lane
-    const bool leader = j == 0;
+    const uint32_t j = lane & 31u;
     const uint32_t n = (uint32_t)p.n;
     const uint32_t W = p.W;
     const uint32_t loops = 0xFFFFFFFFu / W;
@@ -139,17 +361,18 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
     B.lab0 = p.lab + (uint64_t)blockIdx.x * SL * W;
     B.mark0 = p.mark + (uint64_t)blockIdx.x * SL * W;
     B.mask0 = p.mask + (uint64_t)blockIdx.x * 2 * W;
-    B.qg0 = p.qg + (uint64_t)blockIdx.x * 2 * W;
-    B.ql = qlds;
+    B.q0 = p.qg + (uint64_t)blockIdx.x * 2 * W;
     uint32_t off = p.bstate[2 * blockIdx.x];
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
     unsigned long long levels = 0, tot_te = 0, tot_reach = 0;
-    uint64_t* fb = S.fbuf[w];
+    uint32_t* fb = S.fbuf[w];
     uint32_t fcount = 0;
+    for (uint32_t i = tid; i < (uint32_t)(T * SL); i += BB) (&S.cell[0][0])[i] = INF;
+    for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
-        S.cursor = 0;
+        S.cursor = S.tcnt = S.held = 0;
     }
     for (;;) {
         if (tid == 0) S.chunk = atomicAdd(&p.ctrl->src_next, 1u);
@@ -158,7 +381,7 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
         if (ck >= p.n_chunks) break;
         const BlkChunk C = p.chunks[ck];
         const uint32_t nsrc = C.count;
-        const uint32_t wb = p.mat_base[C.mat];  // window base: vertex wb is local 0
+        B.wb = p.mat_base[C.mat];
         if (tid < (uint32_t)SL) {
             S.src[tid] = tid < nsrc ? C.first - tid * C.stride : 0u;
             S.s_te[tid] = 0;
@@ -170,162 +393,78 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
             since = 0;
         }
         __syncthreads();
-        const uint32_t stale = off + W;
-        // this lane's source (local id); lanes of slots >= nsrc never get a mask bit
+        B.off = off;
+        B.stale = off + W;
+        // this lane's source (window-local); lanes of slots >= nsrc never
+        // receive a mask bit
         const uint32_t my_srcg = S.src[j];
-        const uint32_t my_src = my_srcg - wb;
+        const uint32_t my_src = my_srcg - B.wb;
         const uint32_t my_stamp = p.stamp_base + (n - my_srcg);
 
-        // ---- seeding (symbfact.cuh:639-721): wave w seeds slots w, w+BW, ...
-        for (uint32_t k = w; k < nsrc; k += BW) {
-            const uint32_t srcg = S.src[k];
-            const uint32_t src = srcg - wb;
-            const uint32_t stamp = p.stamp_base + (n - srcg);
-            const int32_t b = p.rp[srcg], e = p.rp[srcg + 1];
-            if (lane == 0) {
-                B.lab0[(uint64_t)src * SL + k] = off;
-                atomicAdd(&S.s_te[k], (unsigned long long)(e - b));
-                t.checks += (uint32_t)(e - b);
-            }
-            for (int32_t i = b + (int32_t)lane; i < e; i += WAVE) {
-                const uint32_t mg = (uint32_t)p.ci[i];
-                const uint32_t m = mg - wb;
-                B.mark0[(uint64_t)m * SL + k] = stamp;
-                if (m < src) {
-                    const uint32_t old = B.lab0[(uint64_t)m * SL + k];
-                    if (old >= stale) {
-                        atomicAdd(&S.s_reach[k], 1ull);
-                        atomicAdd(&S.s_te[k], (unsigned long long)(p.rp[mg + 1] - p.rp[mg]));
+        // ---- seeding (symbfact.cuh:639-721) by wave 0 alone, slot after slot,
+        // so the mask read-modify-writes of different slots cannot race
+        if (w == 0) {
+            for (uint32_t k = 0; k < nsrc; k++) {
+                const uint32_t srcg = S.src[k];
+                const uint32_t src = srcg - B.wb;
+                const uint32_t stamp = p.stamp_base + (n - srcg);
+                const int32_t b = p.rp[srcg], e = p.rp[srcg + 1];
+                unsigned long long te = lane == 0 ? (unsigned long long)(e - b) : 0ull, rc = 0;
+                if (lane == 0) {
+                    B.lab0[(uint64_t)src * SL + k] = off;
+                    t.checks += (uint32_t)(e - b);
+                }
+                for (int32_t i = b + (int32_t)lane; i < e; i += WAVE) {
+                    const uint32_t mg = (uint32_t)p.ci[i];
+                    const uint32_t m = mg - B.wb;
+                    B.mark0[(uint64_t)m * SL + k] = stamp;
+                    if (m < src) {
+                        const uint32_t old = B.lab0[(uint64_t)m * SL + k];
+                        if (old >= B.stale) {
+                            rc++;
+                            te += (uint32_t)(p.rp[mg + 1] - p.rp[mg]);
+                        }
+                        B.lab0[(uint64_t)m * SL + k] = off;
+                        const uint32_t o = B.mask0[m];  // level-0 buffer
+                        B.mask0[m] = o | (1u << k);
+                        if (o == 0u) B.q0[atomicAdd(&S.qcnt[0], 1u)] = m;
                     }
-                    B.lab0[(uint64_t)m * SL + k] = off;
-                    const uint32_t o = __hip_atomic_fetch_or(B.mask0 + m, 1u << k, __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_AGENT);
-                    if (o == 0u) q_write(B, 0, atomicAdd(&S.qcnt[0], 1u), m);
+                }
+                te = wave_sum64(te);
+                rc = wave_sum64(rc);
+                if (lane == 0) {
+                    S.s_te[k] += te;
+                    S.s_reach[k] += rc;
                 }
             }
         }
         __syncthreads();
 
-        // ---- level loop: relaxation (symbfact.cuh:883-1082), lane = slot
+        // ---- level loop
         uint32_t cur = 0;
         for (;;) {
             const uint32_t size = S.qcnt[cur];
             __syncthreads();  // every thread has the size before it can change
             if (size == 0) break;
             const uint32_t nbuf = cur ^ 1u;
-            uint32_t* maskc = B.mask0 + (uint64_t)cur * W;
-            uint32_t* maskn = B.mask0 + (uint64_t)nbuf * W;
-            // Software pipeline: the queue entry, row bounds, slot mask and
-            // label of the next vertex pair are fetched while the current
-            // pair's edges are probed.
-            uint32_t i0 = 0;
-            if (lane == 0) i0 = atomicAdd(&S.cursor, 2u);
-            i0 = bcast0(i0);
-            bool nhave = false;
-            uint32_t nv = 0, nM = 0, nlv = 0;
-            int32_t nrb = 0, nre = 0;
-            auto prefetch = [&](uint32_t base_i) {
-                const uint32_t qi = base_i + half;
-                nhave = base_i < size && qi < size;
-                nv = 0;
-                nrb = nre = 0;
-                nM = 0;
-                nlv = 0;
-                if (nhave) {
-                    nv = q_read(B, cur, qi);
-                    nrb = p.rp[wb + nv];
-                    nre = p.rp[wb + nv + 1];
-                    nlv = ld_agent(B.lab0 + (uint64_t)nv * SL + j);
-                    if (leader)
-                        nM = __hip_atomic_exchange(maskc + nv, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            HalfVtx hv;
+            hv.held = false;
+            for (;;) {  // rounds
+                combine_phase(B, S, hv, cur, size, t);
+                if (hv.held && j == 0) atomicOr(&S.held, 1u);
+                __syncthreads();
+                // final for this round: the cursor moves only in phase A
+                const bool more = S.held != 0u || S.cursor < size;
+                relax_phase(B, S, nbuf, my_src, my_stamp, fb, fcount, t);
+                __syncthreads();
+                for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
+                if (tid == 0) {
+                    S.tcnt = 0;
+                    S.held = 0;
                 }
-            };
-            if (i0 < size) prefetch(i0);
-            while (i0 < size) {
-                const bool have = nhave;
-                const uint32_t v = nv;
-                const int32_t rb = nrb, re = nre;
-                const uint32_t Mv = nM, lv = nlv;
-                if (lane == 0) i0 = atomicAdd(&S.cursor, 2u);
-                i0 = bcast0(i0);
-                if (i0 < size) prefetch(i0);
-                const uint32_t M = __shfl(Mv, 0, 32);
-                const bool active = have && ((M >> j) & 1u);
-                const uint32_t cost = (v + off) > lv ? (v + off) : lv;  // symbfact.cuh:924
-                const uint32_t deg = have ? (uint32_t)(re - rb) : 0u;
-                const uint32_t d0 = __shfl(deg, 0, WAVE), d1 = __shfl(deg, 32, WAVE);
-                const uint32_t dmax = d0 > d1 ? d0 : d1;
-                for (uint32_t c0 = 0; c0 < dmax; c0 += 32) {
-                    // each half loads up to 32 CSR words of its row at once
-                    uint32_t cm = 0;
-                    if (c0 + j < deg) cm = (uint32_t)p.ci[rb + (int32_t)(c0 + j)] - wb;
-                    const uint32_t lim = dmax - c0 < 32u ? dmax - c0 : 32u;
-                    for (uint32_t k0 = 0; k0 < lim; k0 += KB) {
-                        // KB edges at a time: all label/marker probes in flight
-                        // together, then the atomics, then the bookkeeping
-                        uint32_t mm[KB], val[KB], old[KB];
-                        uint32_t* adr[KB];
-                        bool go[KB], up[KB], need[KB];
-#pragma unroll
-                        for (int u = 0; u < KB; u++) {
-                            mm[u] = __shfl(cm, (int)((k0 + u) & 31u), 32);
-                            go[u] = active && (k0 + u) < lim && (c0 + k0 + u) < deg;
-                            up[u] = mm[u] > my_src;
-                            adr[u] = (up[u] ? B.mark0 : B.lab0) + (uint64_t)mm[u] * SL + j;
-                        }
-#pragma unroll
-                        for (int u = 0; u < KB; u++) val[u] = go[u] ? ld_filter(adr[u]) : 0u;
-#pragma unroll
-                        for (int u = 0; u < KB; u++)
-                            need[u] = go[u] && (up[u] ? val[u] != my_stamp : val[u] > cost);
-#pragma unroll
-                        for (int u = 0; u < KB; u++)
-                            old[u] = need[u] ? (up[u] ? amax(adr[u], my_stamp) : amin(adr[u], cost)) : 0u;
-                        uint32_t r[KB];  // bit0: relaxed below the source, bit1: new fill
-#pragma unroll
-                        for (int u = 0; u < KB; u++) {
-                            r[u] = 0;
-                            if (go[u]) t.checks++;
-                            if (!need[u]) continue;
-                            if (up[u]) {
-                                r[u] = old[u] < my_stamp ? 2u : 0u;
-                            } else if (old[u] > cost) {
-                                const uint32_t m = mm[u];
-                                if (old[u] >= stale) {  // first reach of m from this source
-                                    t.reach++;
-                                    t.te += (uint32_t)(p.rp[wb + m + 1] - p.rp[wb + m]);
-                                }
-                                r[u] = m < my_src ? 1u : 0u;
-                                if (m + off > cost &&
-                                    amax(B.mark0 + (uint64_t)m * SL + j, my_stamp) < my_stamp)
-                                    r[u] |= 2u;
-                            }
-                            if (r[u] & 2u) t.fills++;
-                        }
-                        // slots that must expand m next level: one atomic per
-                        // half and edge, all issued before any result is used
-                        uint32_t bits[KB], prev[KB];
-#pragma unroll
-                        for (int u = 0; u < KB; u++)
-                            bits[u] = (uint32_t)(__ballot(r[u] & 1u) >> (32u * half));
-#pragma unroll
-                        for (int u = 0; u < KB; u++)
-                            prev[u] = (leader && bits[u])
-                                          ? __hip_atomic_fetch_or(maskn + mm[u], bits[u], __ATOMIC_RELAXED,
-                                                                  __HIP_MEMORY_SCOPE_AGENT)
-                                          : 1u;
-#pragma unroll
-                        for (int u = 0; u < KB; u++)
-                            if (leader && bits[u] && prev[u] == 0u)
-                                q_write(B, nbuf, atomicAdd(&S.qcnt[nbuf], 1u), mm[u]);
-#pragma unroll
-                        for (int u = 0; u < KB; u++)
-                            stage_fill(p, fb, fcount, (r[u] & 2u) != 0,
-                                       ((uint64_t)my_srcg << 32) | (mm[u] + wb));
-                    }
-                }
+                __syncthreads();
+                if (!more) break;
             }
-            __syncthreads();
             if (tid == 0) {
                 S.qcnt[cur] = 0;
                 S.cursor = 0;
@@ -334,6 +473,9 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
             cur = nbuf;
             levels++;
         }
+        // staged fills name their slot: flush before the slots change
+        if (fcount) flush_fills_w(p, S, fb, fcount);
+        fcount = 0;
         // per-row TE and |R_<(s)|: lane j's tallies belong to slot j
         if (t.te) atomicAdd(&S.s_te[j], t.te);
         if (t.reach) atomicAdd(&S.s_reach[j], t.reach);
@@ -348,7 +490,6 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
         off -= W;
         since++;
     }
-    if (fcount) flush_fills_w(p, fb, fcount);
     const unsigned long long f = wave_sum64(t.fills), te = wave_sum64(tot_te),
                              rc = wave_sum64(tot_reach), ch = wave_sum64(t.checks);
     if (lane == 0) {

@@ -33,7 +33,8 @@ struct Ctrl {
     unsigned long long snap_orph[2];
 };
 
-constexpr int kBlockSlots = 32;   // source slots per workgroup in block mode
+constexpr int kBlockSlots = 32;      // source slots per workgroup in block mode
+constexpr int kBlkTable = 512;       // distinct relaxation targets per round (LDS table)
 
 // Block-resident mode: each workgroup owns kBlockSlots source slots.
 // Work comes as a table of chunks, processed in table order (decreasing
@@ -59,8 +60,7 @@ struct BlkParams {
     uint32_t* lab;         // nblocks * W * kBlockSlots labels, [vertex][slot]
     uint32_t* mark;        // nblocks * W * kBlockSlots markers, [vertex][slot]
     uint32_t* mask;        // nblocks * 2 * W frontier slot masks
-    uint32_t* qg;          // nblocks * 2 * W queue entries (global tier)
-    uint32_t ql_cap;       // LDS entries per queue buffer
+    uint32_t* qg;          // nblocks * 2 * W frontier queue entries
     uint32_t stamp_base;
     uint64_t* fills;
     uint64_t fill_cap;
