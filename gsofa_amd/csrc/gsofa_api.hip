@@ -264,11 +264,11 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
     }
 }
 
-// Block mode: LDS for two workgroups per CU, as many workgroups as can be
-// co-resident (or as the chunk option asks: chunk = concurrent sources), shrunk
-// until the per-workgroup arrays (272 n bytes each) fit the budget.
+// Block mode: as many workgroups as can be co-resident (or as the chunk
+// option asks: chunk = concurrent sources), shrunk until the per-workgroup
+// arrays (labels [W][32] + two masks + two queues = 144 W bytes) fit the budget.
 int64_t block_bytes(uint32_t nb, uint32_t W) {
-    return (int64_t)nb * ((8ll * kBlockSlots + 16) * W + 8);
+    return (int64_t)nb * ((4ll * kBlockSlots + 16) * W + 8);
 }
 
 // Work table: chunks of up to kBlockSlots sources of one matrix, in the order
@@ -550,7 +550,6 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         }
         const size_t slots = (size_t)P->nblocks * kBlockSlots;
         e |= dalloc(&P->d_lab, slots * (size_t)P->W);
-        e |= dalloc(&P->d_mark, slots * (size_t)P->W);
         e |= dalloc(&P->d_mask, (size_t)P->nblocks * 2 * (size_t)P->W);
         e |= dalloc(&P->d_qgv, (size_t)P->nblocks * 2 * (size_t)P->W);
         e |= dalloc(&P->d_bstate, 2 * (size_t)P->nblocks);
@@ -595,20 +594,19 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         delete P;
         return GSOFA_ERR_ALLOC;
     }
-    const size_t mark_words = P->mode == GSOFA_MODE_BLOCK ? (size_t)P->nblocks * kBlockSlots * P->W
-                                                          : (size_t)P->C * n;
     bool ok = hipMemcpyAsync(P->d_rp, P->h_rp.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
                              P->stream) == hipSuccess;
     if (P->nnz)
         ok = ok && hipMemcpyAsync(P->d_ci, ci.data(), sizeof(int32_t) * P->nnz, hipMemcpyHostToDevice,
                                   P->stream) == hipSuccess;
-    ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * mark_words, P->stream) == hipSuccess;
-    if (P->mode == GSOFA_MODE_GRID)
+    if (P->mode == GSOFA_MODE_GRID) {
+        ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream) == hipSuccess;
         ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * (size_t)P->C * P->orphan_words,
                                   P->stream) == hipSuccess;
-    else
+    } else {
         ok = ok && hipMemsetAsync(P->d_mask, 0, sizeof(uint32_t) * (size_t)P->nblocks * 2 * P->W,
                                   P->stream) == hipSuccess;
+    }
     std::vector<uint32_t> bst;
     if (P->mode == GSOFA_MODE_BLOCK) {
         ok = ok && hipMemsetAsync(P->d_lab, 0xFF, sizeof(uint32_t) * P->lab_words, P->stream) == hipSuccess;
@@ -739,9 +737,9 @@ int gsofa_plan_execute(gsofa_plan* P) {
         for (size_t si = 0; si < shares.size(); si++) {
             const int32_t gid = shares[si];
             const uint64_t cnt = share_count(n, N, gid);
-            // stamps must increase per marker slot: move to a fresh stamp range
-            // (or clear the markers when the 32-bit range is used up)
-            if (!P->marks_clean) {
+            // grid mode: stamps must increase per marker slot: move to a fresh
+            // stamp range (or clear the markers when the 32-bit range is used up)
+            if (P->mode == GSOFA_MODE_GRID && !P->marks_clean) {
                 if ((uint64_t)(P->epoch + 2) * (uint64_t)n >= 0xFFFFFFFFull) {
                     HIPCHK(hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream));
                     P->epoch = 0;
@@ -763,10 +761,8 @@ int gsofa_plan_execute(gsofa_plan* P) {
                 bp.row_reach = P->d_row_reach;
                 bp.nblocks = P->nblocks;
                 bp.lab = P->d_lab;
-                bp.mark = P->d_mark;
                 bp.mask = P->d_mask;
                 bp.qg = P->d_qgv;
-                bp.stamp_base = P->epoch * (uint32_t)n;
                 bp.fills = P->d_fills;
                 bp.fill_cap = P->fill_cap;
                 bp.ctrl = P->d_ctrl;

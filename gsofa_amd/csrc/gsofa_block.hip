@@ -9,7 +9,7 @@
 //   their frontier levels by itself, synchronising only with __syncthreads.
 //   The next mini-group comes from one global counter.  There is no grid-wide
 //   barrier and no hot global counter on the level path.
-// * Labels and markers are stored slot-interleaved, [vertex][32 slots]: the
+// * Labels are stored slot-interleaved, [vertex][32 slots]: the
 //   32 sources' max-id labels of one vertex share one 128-byte line.  Nearby
 //   sources reach mostly the same vertices in the same levels, so the frontier
 //   is kept per vertex with a 32-bit mask of the slots that must expand it
@@ -23,7 +23,7 @@
 //        of the round's distinct targets, and folds the cost into that
 //        target's 32 per-slot cells with an LDS atomic min;
 //     B. relax: every table entry (a distinct target m) is relaxed once per
-//        slot against its label or marker exactly as the reference relaxes
+//        slot against its label exactly as the reference relaxes
 //        one edge (symbfact.cuh:989-1066), with the round's minimum cost.
 //        Targets are unique within a round, so plain loads and stores suffice.
 //   Folding several edges into one relaxation with their minimum cost gives
@@ -33,10 +33,20 @@
 //   A frontier vertex's own label may drop during a later round of the same
 //   level; that drop re-enqueues the vertex, so expanding it with the older
 //   cost is correct (label-correcting).
+// * Fills are read off the labels, so there is no marker array.  The
+//   reference stamps a marker per (source, vertex) to count each fill once
+//   (symbfact.cuh:992-1009, 1046-1063).  Here labels only decrease during a
+//   source's traversal and use the per-group decreasing offset of
+//   symbfact.cuh:2388-2407 (kept per workgroup; older groups' labels compare
+//   as stale), so:
+//     - a vertex m below the source is a new L entry exactly when its label
+//       first drops below m + off (the reference's "m + off > cost" on the
+//       first stamp); neighbours of the source start at off and never count;
+//     - a vertex m above the source is a new U entry exactly when it is first
+//       touched: its label cell (unused as a label for this source) goes
+//       from stale to a current value; neighbours start touched.
 // * Sources are taken in decreasing order so the deepest traversals start
-//   first; marker stamps are stamp_base + (n - s), still increasing per slot,
-//   so markers are never reset (symbfact.cuh:1006).  Labels use the per-group
-//   decreasing offset of symbfact.cuh:2388-2407, kept per workgroup.
+//   first.
 // * Work is a table of chunks (up to 32 sources of one matrix).  A batch of
 //   matrices is one block-diagonal matrix in which no traversal can leave its
 //   own matrix, so per-workgroup arrays are indexed relative to the chunk's
@@ -63,10 +73,14 @@ constexpr uint32_t INF = 0xFFFFFFFFu;
 constexpr uint32_t NONE = 0xFFFFFFFFu;   // no table entry (edge not in this group)
 constexpr uint32_t FULL = 0xFFFFFFFEu;   // table full: retry the edge next round
 constexpr uint32_t HOLE = 0xFFFFFFFFu;   // ent_m of an index that lost its insert
-#ifndef GSOFA_KB
-#define GSOFA_KB 4
+#ifndef GSOFA_KR
+#define GSOFA_KR 4
 #endif
-constexpr int KR = GSOFA_KB;         // table entries relaxed together per half-wave
+#ifndef GSOFA_KV
+#define GSOFA_KV 4
+#endif
+constexpr int KR = GSOFA_KR;         // table entries relaxed together per half-wave
+constexpr int KV = GSOFA_KV;         // frontier vertices fetched together per half-wave
 static_assert(T <= H / 2, "hash load factor");
 
 struct BlkLds {
@@ -88,13 +102,24 @@ struct BlkLds {
 struct Blk {
     const BlkParams* p;
     uint32_t* lab0;    // [W][SL]
-    uint32_t* mark0;   // [W][SL]
     uint32_t* mask0;   // [2][W]
     uint32_t* q0;      // [2][W]
     uint32_t W;
     uint32_t wb;       // window base (global id of local vertex 0)
     uint32_t off, stale;
 };
+
+// Element at a 32-bit byte offset from a workgroup-uniform base: the offset
+// stays one VGPR and the load/store uses the scalar-base addressing form.
+// Every per-workgroup array here is far below 4 GiB.
+template <class T>
+__device__ __forceinline__ T& at(T* base, uint32_t byte_off) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ const T& at(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
 
 struct TallyB {
     unsigned long long fills, te, reach, checks;
@@ -168,13 +193,20 @@ __device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
     return res;
 }
 
-// Frontier vertex held by a half-wave across rounds.  All lanes of the half
-// hold the same vertex; lane j holds slot j's cost.
+// Frontier vertices held by a half-wave across rounds: up to KV vertices are
+// fetched together so their dependent loads overlap.  All lanes of the half
+// hold the same vertices; lane j holds slot j's cost of each.
 struct HalfVtx {
     bool held;
-    uint32_t v, M, cost, deg, done, pend;
-    int32_t rb;
+    uint32_t cost[KV];  // lane j: max(v + off, label_j(v)), INF if slot j is not in v's mask
+    uint32_t pend[KV];  // edges of the current 32-edge group not yet folded
+    int32_t pos[KV];    // CSR index of the current group's first edge
+    int32_t end[KV];    // CSR end of the row
 };
+
+__device__ __forceinline__ uint32_t group_mask(int32_t left) {
+    return left >= 32 ? 0xFFFFFFFFu : (left <= 0 ? 0u : ((1u << left) - 1u));
+}
 
 // Phase A: pop frontier vertices and fold their edges' costs into the table
 // until the queue is exhausted or the table is full.  Control flow diverges
@@ -191,56 +223,81 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
     for (;;) {
         if (!hv.held) {
             if (S.tcnt >= (uint32_t)T) return;  // table full: take no new vertex
-            uint32_t i = 0;
-            if (j == 0) i = atomicAdd(&S.cursor, 1u);
-            i = __shfl(i, (int)(lane & 32u), WAVE);
-            if (i >= size) return;
-            hv.v = qc[i];
-            hv.M = maskc[hv.v];
-            if (j == 0) maskc[hv.v] = 0u;  // consumed; this buffer is refilled two levels later
-            hv.rb = p.rp[B.wb + hv.v];
-            hv.deg = (uint32_t)(p.rp[B.wb + hv.v + 1] - hv.rb);
-            hv.cost = INF;
-            if ((hv.M >> j) & 1u) {
-                const uint32_t lv = B.lab0[(uint64_t)hv.v * SL + j];
-                const uint32_t vo = hv.v + B.off;
-                hv.cost = vo > lv ? vo : lv;  // symbfact.cuh:924
+            uint32_t i0 = 0;
+            if (j == 0) i0 = atomicAdd(&S.cursor, (uint32_t)KV);
+            i0 = __shfl(i0, (int)(lane & 32u), WAVE);
+            if (i0 >= size) return;
+            uint32_t vv[KV], M[KV], lv[KV];
+            int32_t r0[KV], r1[KV];
+#pragma unroll
+            for (int u = 0; u < KV; u++) vv[u] = i0 + u < size ? at(qc, (i0 + u) << 2) : NONE;
+#pragma unroll
+            for (int u = 0; u < KV; u++) {  // independent loads, issued together
+                const bool ok = vv[u] != NONE;
+                M[u] = ok ? at(maskc, vv[u] << 2) : 0u;
+                r0[u] = ok ? at(p.rp, (B.wb + vv[u]) << 2) : 0;
+                r1[u] = ok ? at(p.rp, (B.wb + vv[u] + 1) << 2) : 0;
+                lv[u] = ok ? at(B.lab0, (vv[u] * SL + j) << 2) : 0u;
             }
-            if (j == 0) t.checks += (unsigned long long)hv.deg * (uint32_t)__popc(hv.M);
-            hv.done = 0;
-            hv.pend = hv.deg >= 32u ? 0xFFFFFFFFu : ((1u << hv.deg) - 1u);
+#pragma unroll
+            for (int u = 0; u < KV; u++) {
+                // consumed; this buffer is refilled two levels later
+                if (j == 0 && vv[u] != NONE) at(maskc, vv[u] << 2) = 0u;
+                const uint32_t vo = vv[u] + B.off;
+                hv.cost[u] = ((M[u] >> j) & 1u) ? (vo > lv[u] ? vo : lv[u]) : INF;  // symbfact.cuh:924
+                hv.pos[u] = r0[u];
+                hv.end[u] = r1[u];
+                hv.pend[u] = group_mask(r1[u] - r0[u]);
+                if (j == 0) t.checks += (unsigned long long)(uint32_t)(r1[u] - r0[u]) * (uint32_t)__popc(M[u]);
+            }
             hv.held = true;
         }
-        // one group of up to 32 edges; pend = edges of the group not yet folded
-        const bool mine = (hv.pend >> j) & 1u;
-        uint32_t idx = NONE;
-        if (mine) idx = tab_find(S, (uint32_t)p.ci[hv.rb + (int32_t)(hv.done + j)] - B.wb);
-        S.eidx[h][j] = idx;
-        const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
-        wave_sync();
-        if (hv.cost != INF) {
-            for (uint32_t b = okm; b; b &= b - 1u) {
-                const uint32_t id = S.eidx[h][__builtin_ctz(b)];
-                atomicMin(&S.cell[id][j], hv.cost);
+        // one group of up to 32 edges of every held vertex
+        uint32_t mv[KV];
+#pragma unroll
+        for (int u = 0; u < KV; u++)
+            mv[u] = ((hv.pend[u] >> j) & 1u) ? (uint32_t)at(p.ci, (uint32_t)(hv.pos[u] + (int32_t)j) << 2) - B.wb : 0u;
+        bool full = false;
+#pragma unroll
+        for (int u = 0; u < KV; u++) {
+            const bool mine = (hv.pend[u] >> j) & 1u;
+            uint32_t idx = NONE;
+            if (mine) idx = tab_find(S, mv[u]);
+            S.eidx[h][j] = idx;
+            const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
+            full = full || (okm != hv.pend[u]);
+            wave_sync();
+            if (hv.cost[u] != INF) {
+                for (uint32_t b = okm; b; b &= b - 1u) {
+                    const uint32_t id = S.eidx[h][__builtin_ctz(b)];
+                    atomicMin(&S.cell[id][j], hv.cost[u]);
+                }
             }
+            wave_sync();
+            hv.pend[u] &= ~okm;
         }
-        wave_sync();
-        hv.pend &= ~okm;
-        if (hv.pend) return;  // table full: the rest of the group waits
-        hv.done += 32u;
-        if (hv.done >= hv.deg) {
-            hv.held = false;
-        } else {
-            const uint32_t r = hv.deg - hv.done;
-            hv.pend = r >= 32u ? 0xFFFFFFFFu : ((1u << r) - 1u);
+        if (full) return;  // table full: the rest waits for the next round
+        bool any = false;
+#pragma unroll
+        for (int u = 0; u < KV; u++) {
+            if (hv.pos[u] + 32 < hv.end[u]) {  // next group of a long row
+                hv.pos[u] += 32;
+                hv.pend[u] = group_mask(hv.end[u] - hv.pos[u]);
+            } else {
+                hv.pos[u] = hv.end[u];
+            }
+            any = any || hv.pend[u] != 0u;
         }
+        if (!any) hv.held = false;
     }
 }
 
 // Phase B: relax every table entry once per slot.  Wave-uniform trip count.
+// The degree of a target below the source is loaded with its label, whether
+// or not the relaxation turns out to be a first reach, so each batch of KR
+// entries costs one memory round trip.
 __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_src,
-                            uint32_t my_stamp, uint32_t* fb, uint32_t& fcount,
-                            TallyB& t) {
+                            uint32_t* fb, uint32_t& fcount, TallyB& t) {
     const BlkParams& p = *B.p;
     const uint32_t lane = __lane_id();
     const uint32_t half = lane >> 5;
@@ -254,6 +311,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
     const uint32_t off = B.off;
     for (uint32_t base = w * 2 * KR; base < ntab; base += BW * 2 * KR) {
         uint32_t mm[KR], c[KR], lm[KR], nmk[KR];
+        int32_t d0[KR], d1[KR];
         bool ok[KR], act[KR], up[KR];
 #pragma unroll
         for (int u = 0; u < KR; u++) {
@@ -269,57 +327,37 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
             act[u] = c[u] != INF;
             up[u] = mm[u] > my_src;
         }
-        // round trip 1: the target's label (or marker above the source) and
-        // its next-level slot mask
+        // the one round trip: label, degree, next-level slot mask
+        uint32_t o[KR];
 #pragma unroll
         for (int u = 0; u < KR; u++) {
-            lm[u] = act[u] ? (up[u] ? B.mark0 : B.lab0)[(uint64_t)mm[u] * SL + j] : 0u;
-            nmk[u] = (ok[u] && leader) ? maskn[mm[u]] : 0u;
+            const bool lo = act[u] && !up[u];
+            o[u] = (mm[u] * SL + j) << 2;
+            const uint32_t ro = (B.wb + mm[u]) << 2;
+            lm[u] = act[u] ? at(B.lab0, o[u]) : 0u;
+            d0[u] = lo ? at(p.rp, ro) : 0;
+            d1[u] = lo ? at(p.rp, ro + 4u) : 0;
+            nmk[u] = (ok[u] && leader) ? at(maskn, mm[u] << 2) : 0u;
         }
-        bool first[KR], lcand[KR], push[KR], ufill[KR];
+        bool push[KR], fill[KR];
 #pragma unroll
         for (int u = 0; u < KR; u++) {
-            first[u] = lcand[u] = push[u] = ufill[u] = false;
+            push[u] = fill[u] = false;
             if (act[u]) {
-                if (up[u]) {  // m above the source: U entry (symbfact.cuh:992-1009)
-                    if (lm[u] != my_stamp) {
-                        B.mark0[(uint64_t)mm[u] * SL + j] = my_stamp;
-                        ufill[u] = true;
+                if (up[u]) {  // m above the source: U entry on first touch (symbfact.cuh:992-1009)
+                    if (lm[u] >= B.stale) {
+                        at(B.lab0, o[u]) = c[u];
+                        fill[u] = true;
                     }
                 } else if (lm[u] > c[u]) {  // relaxation (symbfact.cuh:1018-1063)
-                    B.lab0[(uint64_t)mm[u] * SL + j] = c[u];
-                    first[u] = lm[u] >= B.stale;
+                    at(B.lab0, o[u]) = c[u];
                     push[u] = mm[u] < my_src;
-                    lcand[u] = mm[u] + off > c[u];
-                }
-            }
-        }
-        // round trip 2 (only when some lane needs it): L-fill markers and the
-        // degrees of first-reached vertices (TE_alg)
-        bool need2 = false;
-#pragma unroll
-        for (int u = 0; u < KR; u++) need2 = need2 || lcand[u] || first[u];
-        bool lfill[KR];
-#pragma unroll
-        for (int u = 0; u < KR; u++) lfill[u] = false;
-        if (__ballot(need2)) {
-            uint32_t mk[KR];
-            int32_t d0[KR], d1[KR];
-#pragma unroll
-            for (int u = 0; u < KR; u++) {
-                mk[u] = lcand[u] ? B.mark0[(uint64_t)mm[u] * SL + j] : 0u;
-                d0[u] = first[u] ? p.rp[B.wb + mm[u]] : 0;
-                d1[u] = first[u] ? p.rp[B.wb + mm[u] + 1] : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < KR; u++) {
-                if (lcand[u] && mk[u] < my_stamp) {
-                    B.mark0[(uint64_t)mm[u] * SL + j] = my_stamp;
-                    lfill[u] = true;
-                }
-                if (first[u]) {
-                    t.reach++;
-                    t.te += (uint32_t)(d1[u] - d0[u]);
+                    if (lm[u] >= B.stale) {  // first reach: |R_<(s)| and TE_alg
+                        t.reach++;
+                        t.te += (uint32_t)(d1[u] - d0[u]);
+                    }
+                    const uint32_t mo = mm[u] + off;
+                    fill[u] = lm[u] >= mo && c[u] < mo;  // label crosses below m: new L entry
                 }
             }
         }
@@ -332,16 +370,15 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
 #pragma unroll
             for (int u = 0; u < KR; u++) {
                 if (bits[u] && (nmk[u] | bits[u]) != nmk[u]) {
-                    maskn[mm[u]] = nmk[u] | bits[u];
+                    at(maskn, mm[u] << 2) = nmk[u] | bits[u];
                     if (nmk[u] == 0u) qn[atomicAdd(&S.qcnt[nbuf], 1u)] = mm[u];
                 }
             }
         }
 #pragma unroll
         for (int u = 0; u < KR; u++) {
-            const bool fill = ufill[u] || lfill[u];
-            if (fill) t.fills++;
-            stage_fill(p, S, fb, fcount, fill, (j << 25) | (mm[u] + B.wb));
+            if (fill[u]) t.fills++;
+            stage_fill(p, S, fb, fcount, fill[u], (j << 25) | (mm[u] + B.wb));
         }
     }
 }
@@ -352,14 +389,12 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / WAVE;
     const uint32_t j = lane & 31u;
-    const uint32_t n = (uint32_t)p.n;
     const uint32_t W = p.W;
     const uint32_t loops = 0xFFFFFFFFu / W;
     Blk B;
     B.p = &p;
     B.W = W;
     B.lab0 = p.lab + (uint64_t)blockIdx.x * SL * W;
-    B.mark0 = p.mark + (uint64_t)blockIdx.x * SL * W;
     B.mask0 = p.mask + (uint64_t)blockIdx.x * 2 * W;
     B.q0 = p.qg + (uint64_t)blockIdx.x * 2 * W;
     uint32_t off = p.bstate[2 * blockIdx.x];
@@ -397,37 +432,37 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
         B.stale = off + W;
         // this lane's source (window-local); lanes of slots >= nsrc never
         // receive a mask bit
-        const uint32_t my_srcg = S.src[j];
-        const uint32_t my_src = my_srcg - B.wb;
-        const uint32_t my_stamp = p.stamp_base + (n - my_srcg);
+        const uint32_t my_src = S.src[j] - B.wb;
 
         // ---- seeding (symbfact.cuh:639-721) by wave 0 alone, slot after slot,
-        // so the mask read-modify-writes of different slots cannot race
+        // so the mask read-modify-writes of different slots cannot race.
+        // Every neighbour's label becomes off: below the source that is the
+        // reference's seed label, above it the "touched" state; either way
+        // neighbours never count as fills (the reference's pre-stamp).
         if (w == 0) {
             for (uint32_t k = 0; k < nsrc; k++) {
                 const uint32_t srcg = S.src[k];
                 const uint32_t src = srcg - B.wb;
-                const uint32_t stamp = p.stamp_base + (n - srcg);
                 const int32_t b = p.rp[srcg], e = p.rp[srcg + 1];
                 unsigned long long te = lane == 0 ? (unsigned long long)(e - b) : 0ull, rc = 0;
                 if (lane == 0) {
-                    B.lab0[(uint64_t)src * SL + k] = off;
+                    at(B.lab0, (src * SL + k) << 2) = off;
                     t.checks += (uint32_t)(e - b);
                 }
                 for (int32_t i = b + (int32_t)lane; i < e; i += WAVE) {
                     const uint32_t mg = (uint32_t)p.ci[i];
                     const uint32_t m = mg - B.wb;
-                    B.mark0[(uint64_t)m * SL + k] = stamp;
+                    const uint32_t o = (m * SL + k) << 2;
+                    const uint32_t old = at(B.lab0, o);
+                    at(B.lab0, o) = off;
                     if (m < src) {
-                        const uint32_t old = B.lab0[(uint64_t)m * SL + k];
                         if (old >= B.stale) {
                             rc++;
                             te += (uint32_t)(p.rp[mg + 1] - p.rp[mg]);
                         }
-                        B.lab0[(uint64_t)m * SL + k] = off;
-                        const uint32_t o = B.mask0[m];  // level-0 buffer
-                        B.mask0[m] = o | (1u << k);
-                        if (o == 0u) B.q0[atomicAdd(&S.qcnt[0], 1u)] = m;
+                        const uint32_t mo = B.mask0[m];  // level-0 buffer
+                        B.mask0[m] = mo | (1u << k);
+                        if (mo == 0u) B.q0[atomicAdd(&S.qcnt[0], 1u)] = m;
                     }
                 }
                 te = wave_sum64(te);
@@ -455,7 +490,7 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
                 const bool more = S.held != 0u || S.cursor < size;
-                relax_phase(B, S, nbuf, my_src, my_stamp, fb, fcount, t);
+                relax_phase(B, S, nbuf, my_src, fb, fcount, t);
                 __syncthreads();
                 for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
                 if (tid == 0) {

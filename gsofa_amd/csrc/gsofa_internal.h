@@ -58,10 +58,8 @@ struct BlkParams {
     const uint32_t* mat_base;  // n_mats + 1 vertex offsets
     uint32_t nblocks;
     uint32_t* lab;         // nblocks * W * kBlockSlots labels, [vertex][slot]
-    uint32_t* mark;        // nblocks * W * kBlockSlots markers, [vertex][slot]
     uint32_t* mask;        // nblocks * 2 * W frontier slot masks
     uint32_t* qg;          // nblocks * 2 * W frontier queue entries
-    uint32_t stamp_base;
     uint64_t* fills;
     uint64_t fill_cap;
     Ctrl* ctrl;
