@@ -35,7 +35,7 @@ struct gsofa_plan {
     uint32_t ql_cap = 0, qg_cap = 0;
     size_t blk_lds = 0;
     uint32_t* d_mask = nullptr;
-    uint32_t* d_qgv = nullptr;
+    uint4* d_qgv = nullptr;
     uint32_t* d_bstate = nullptr;
     // matrices of a batch (block-diagonal global numbering) and the work table
     int32_t n_mats = 1;
@@ -266,9 +266,10 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
 
 // Block mode: as many workgroups as can be co-resident (or as the chunk
 // option asks: chunk = concurrent sources), shrunk until the per-workgroup
-// arrays (labels [W][32] + two masks + two queues = 144 W bytes) fit the budget.
+// arrays (labels [W][32] + two masks + two 16-byte-entry queues = 168 W
+// bytes) fit the budget.
 int64_t block_bytes(uint32_t nb, uint32_t W) {
-    return (int64_t)nb * ((4ll * kBlockSlots + 16) * W + 8);
+    return (int64_t)nb * ((4ll * kBlockSlots + 8 + 32) * W + 8);
 }
 
 // Work table: chunks of up to kBlockSlots sources of one matrix, in the order

@@ -61,7 +61,10 @@ namespace {
 
 using namespace dev;
 
-constexpr int BB = 512;              // threads per workgroup
+#ifndef GSOFA_BB
+#define GSOFA_BB 512
+#endif
+constexpr int BB = GSOFA_BB;         // threads per workgroup
 constexpr int BW = BB / WAVE;        // waves per workgroup
 constexpr int NH = 2 * BW;           // half-waves
 constexpr int SL = kBlockSlots;      // slots per workgroup (lanes of a half-wave)
@@ -74,20 +77,21 @@ constexpr uint32_t NONE = 0xFFFFFFFFu;   // no table entry (edge not in this gro
 constexpr uint32_t FULL = 0xFFFFFFFEu;   // table full: retry the edge next round
 constexpr uint32_t HOLE = 0xFFFFFFFFu;   // ent_m of an index that lost its insert
 #ifndef GSOFA_KR
-#define GSOFA_KR 4
+#define GSOFA_KR 16
 #endif
 #ifndef GSOFA_KV
 #define GSOFA_KV 4
 #endif
-constexpr int KR = GSOFA_KR;         // table entries relaxed together per half-wave
-constexpr int KV = GSOFA_KV;         // frontier vertices fetched together per half-wave
+constexpr int KR = GSOFA_KR;         // table entries relaxed together per half-wave (<= 32)
+constexpr int KV = GSOFA_KV;         // frontier vertices fetched together per half-wave (<= 32)
+static_assert(KR <= 32 && KV <= 32, "one entry / vertex per lane of a half-wave");
 static_assert(T <= H / 2, "hash load factor");
 
 struct BlkLds {
     uint32_t cell[T][SL];            // min cost per (round target, slot); INF = none
     unsigned long long hkey[H];      // ((m + 1) << 32) | table index; 0 = empty
     uint32_t ent_m[T];               // table index -> target (window-local) or HOLE
-    uint32_t eidx[NH][32];           // table index of each lane's edge
+    uint4 eidx[NH][8];               // table index of each lane's edge (32 per half-wave)
     uint32_t fbuf[BW][FB];           // staged fills, see flush_fills_w
     unsigned long long s_te[SL];     // per-slot TE of the current mini-group
     unsigned long long s_reach[SL];  // per-slot |R_<(s)|
@@ -103,7 +107,7 @@ struct Blk {
     const BlkParams* p;
     uint32_t* lab0;    // [W][SL]
     uint32_t* mask0;   // [2][W]
-    uint32_t* q0;      // [2][W]
+    uint4* q0;         // [2][W] frontier entries (v, row begin, row end, -)
     uint32_t W;
     uint32_t wb;       // window base (global id of local vertex 0)
     uint32_t off, stale;
@@ -193,24 +197,34 @@ __device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
     return res;
 }
 
-// Frontier vertices held by a half-wave across rounds: up to KV vertices are
-// fetched together so their dependent loads overlap.  All lanes of the half
-// hold the same vertices; lane j holds slot j's cost of each.
+// Lane u's value of x within the caller's half-wave (u uniform).  Reads both
+// candidate lanes regardless of EXEC, so x must be written in every lane.
+__device__ __forceinline__ uint32_t from_lane(uint32_t x, uint32_t u, uint32_t half) {
+    const uint32_t a = __builtin_amdgcn_readlane(x, u);
+    const uint32_t b = __builtin_amdgcn_readlane(x, u + 32u);
+    return half ? b : a;
+}
+
+// Frontier vertices held by a half-wave across rounds.  Lane k of the half
+// keeps vertex k's CSR cursor (pos, end) and its pending-edge mask for the
+// current 32-edge group; lane j keeps slot j's cost of every vertex and the
+// j-th edge target of its current group.
 struct HalfVtx {
-    bool held;
-    uint32_t cost[KV];  // lane j: max(v + off, label_j(v)), INF if slot j is not in v's mask
-    uint32_t pend[KV];  // edges of the current 32-edge group not yet folded
-    int32_t pos[KV];    // CSR index of the current group's first edge
-    int32_t end[KV];    // CSR end of the row
+    bool held;           // half-uniform: a batch is in progress
+    bool done;           // half-uniform: the queue is exhausted for this level
+    int32_t pos, end;    // lane k: vertex k's current group start, row end
+    uint32_t pend;       // lane k: vertex k's edges of the group not yet folded
+    uint32_t cost[KV];   // lane j: max(v_u + off, label_j(v_u)), INF if j not in v_u's mask
+    uint32_t civ[KV];    // lane j: target of edge j of v_u's current group (window-local)
 };
 
 __device__ __forceinline__ uint32_t group_mask(int32_t left) {
     return left >= 32 ? 0xFFFFFFFFu : (left <= 0 ? 0u : ((1u << left) - 1u));
 }
 
-// Phase A: pop frontier vertices and fold their edges' costs into the table
-// until the queue is exhausted or the table is full.  Control flow diverges
-// between the two halves of a wave; nothing here needs the whole wave.
+// Phase A: pop frontier vertices (KV per half-wave) and fold their edges'
+// costs into the table until the queue is exhausted or the table is full.
+// Wave-uniform control flow; the two halves carry independent batches.
 __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur, uint32_t size,
                               TallyB& t) {
     const BlkParams& p = *B.p;
@@ -219,171 +233,197 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
     const uint32_t j = lane & 31u;
     const uint32_t h = (threadIdx.x / WAVE) * 2 + half;
     uint32_t* maskc = B.mask0 + (uint64_t)cur * B.W;
-    const uint32_t* qc = B.q0 + (uint64_t)cur * B.W;
+    const uint4* qc = B.q0 + (uint64_t)cur * B.W;
+    if (hv.held) {  // resuming after a full table: reload the pending groups' targets
+#pragma unroll
+        for (int u = 0; u < KV; u++) {
+            const uint32_t pu = from_lane(hv.pend, u, half);
+            const int32_t pos = (int32_t)from_lane((uint32_t)hv.pos, u, half);
+            hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
+        }
+    }
     for (;;) {
-        if (!hv.held) {
-            if (S.tcnt >= (uint32_t)T) return;  // table full: take no new vertex
-            uint32_t i0 = 0;
-            if (j == 0) i0 = atomicAdd(&S.cursor, (uint32_t)KV);
-            i0 = __shfl(i0, (int)(lane & 32u), WAVE);
-            if (i0 >= size) return;
-            uint32_t vv[KV], M[KV], lv[KV];
-            int32_t r0[KV], r1[KV];
+        // -- take a new batch where a half has none
+        const bool want = !hv.held && !hv.done && S.tcnt < (uint32_t)T;
+        uint32_t i0 = 0;
+        if (j == 0 && want) i0 = atomicAdd(&S.cursor, (uint32_t)KV);
+        i0 = from_lane(i0, 0, half);
+        if (want && i0 >= size) hv.done = true;
+        const bool fresh = want && i0 < size;
+        if (__ballot(fresh)) {
+            // lane k: queue entry (v, row begin, row end) and slot mask of vertex k
+            const bool mine = fresh && j < (uint32_t)KV && i0 + j < size;
+            uint4 q = make_uint4(NONE, 0u, 0u, 0u);
+            if (mine) q = at(qc, (i0 + j) << 4);
+            uint32_t M = 0;
+            if (mine) M = at(maskc, q.x << 2);
+            if (fresh) {
+                hv.pos = (int32_t)q.y;
+                hv.end = (int32_t)q.z;
+                hv.pend = group_mask((int32_t)(q.z - q.y));
+            }
+            // every vertex's label line and first edge group, issued together
+            uint32_t lv[KV];
 #pragma unroll
-            for (int u = 0; u < KV; u++) vv[u] = i0 + u < size ? at(qc, (i0 + u) << 2) : NONE;
-#pragma unroll
-            for (int u = 0; u < KV; u++) {  // independent loads, issued together
-                const bool ok = vv[u] != NONE;
-                M[u] = ok ? at(maskc, vv[u] << 2) : 0u;
-                r0[u] = ok ? at(p.rp, (B.wb + vv[u]) << 2) : 0;
-                r1[u] = ok ? at(p.rp, (B.wb + vv[u] + 1) << 2) : 0;
-                lv[u] = ok ? at(B.lab0, (vv[u] * SL + j) << 2) : 0u;
+            for (int u = 0; u < KV; u++) {
+                const uint32_t vu = from_lane(q.x, u, half);
+                const uint32_t pu = from_lane(hv.pend, u, half);
+                const int32_t pos = (int32_t)from_lane((uint32_t)hv.pos, u, half);
+                lv[u] = 0u;
+                if (fresh && vu != NONE) lv[u] = at(B.lab0, (vu * SL + j) << 2);
+                if (fresh) hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
+            }
+            if (mine) {
+                at(maskc, q.x << 2) = 0u;  // consumed; this buffer is refilled two levels later
+                t.checks += (unsigned long long)(q.z - q.y) * (uint32_t)__popc(M);
             }
 #pragma unroll
             for (int u = 0; u < KV; u++) {
-                // consumed; this buffer is refilled two levels later
-                if (j == 0 && vv[u] != NONE) at(maskc, vv[u] << 2) = 0u;
-                const uint32_t vo = vv[u] + B.off;
-                hv.cost[u] = ((M[u] >> j) & 1u) ? (vo > lv[u] ? vo : lv[u]) : INF;  // symbfact.cuh:924
-                hv.pos[u] = r0[u];
-                hv.end[u] = r1[u];
-                hv.pend[u] = group_mask(r1[u] - r0[u]);
-                if (j == 0) t.checks += (unsigned long long)(uint32_t)(r1[u] - r0[u]) * (uint32_t)__popc(M[u]);
+                const uint32_t vu = from_lane(q.x, u, half);
+                const uint32_t Mu = from_lane(M, u, half);
+                const uint32_t vo = vu + B.off;
+                if (fresh)  // symbfact.cuh:924
+                    hv.cost[u] = ((Mu >> j) & 1u) && vu != NONE ? (vo > lv[u] ? vo : lv[u]) : INF;
             }
-            hv.held = true;
+            if (fresh) hv.held = true;
         }
-        // one group of up to 32 edges of every held vertex
-        uint32_t mv[KV];
-#pragma unroll
-        for (int u = 0; u < KV; u++)
-            mv[u] = ((hv.pend[u] >> j) & 1u) ? (uint32_t)at(p.ci, (uint32_t)(hv.pos[u] + (int32_t)j) << 2) - B.wb : 0u;
+        if (!__ballot(hv.held)) return;  // nothing left for either half
+        // -- fold the held vertices' edges, vertex by vertex
         bool full = false;
 #pragma unroll
         for (int u = 0; u < KV; u++) {
-            const bool mine = (hv.pend[u] >> j) & 1u;
-            uint32_t idx = NONE;
-            if (mine) idx = tab_find(S, mv[u]);
-            S.eidx[h][j] = idx;
-            const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
-            full = full || (okm != hv.pend[u]);
-            wave_sync();
-            if (hv.cost[u] != INF) {
-                for (uint32_t b = okm; b; b &= b - 1u) {
-                    const uint32_t id = S.eidx[h][__builtin_ctz(b)];
-                    atomicMin(&S.cell[id][j], hv.cost[u]);
+            uint32_t pu = hv.held ? from_lane(hv.pend, u, half) : 0u;
+            while (pu) {  // half-uniform
+                const bool mine = (pu >> j) & 1u;
+                uint32_t idx = NONE;
+                if (mine) idx = tab_find(S, hv.civ[u]);
+                reinterpret_cast<uint32_t*>(S.eidx[h])[j] = idx;
+                const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
+                wave_sync();
+                if (hv.cost[u] != INF && okm) {
+                    // four indices per LDS read; FULL edges (okm clear) are skipped
+                    const uint32_t top = 32u - (uint32_t)__builtin_clz(okm);
+                    for (uint32_t k4 = 0; k4 < top; k4 += 4) {
+                        const uint4 id = S.eidx[h][k4 >> 2];
+                        const uint32_t bits = okm >> k4;
+                        if (bits & 1u) atomicMin(&S.cell[id.x][j], hv.cost[u]);
+                        if (bits & 2u) atomicMin(&S.cell[id.y][j], hv.cost[u]);
+                        if (bits & 4u) atomicMin(&S.cell[id.z][j], hv.cost[u]);
+                        if (bits & 8u) atomicMin(&S.cell[id.w][j], hv.cost[u]);
+                    }
+                }
+                wave_sync();
+                pu &= ~okm;
+                if (pu) {  // table full: keep the rest of this group
+                    full = true;
+                    break;
+                }
+                // next group of a long row (rare): load its targets now
+                int32_t pos = (int32_t)from_lane((uint32_t)hv.pos, u, half);
+                const int32_t end = (int32_t)from_lane((uint32_t)hv.end, u, half);
+                if (pos + 32 < end) {
+                    pos += 32;
+                    pu = group_mask(end - pos);
+                    if (j == (uint32_t)u) hv.pos = pos;
+                    hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
                 }
             }
-            wave_sync();
-            hv.pend[u] &= ~okm;
+            if (hv.held && j == (uint32_t)u) hv.pend = pu;
+            if (__ballot(full)) return;  // the table is full for everyone
         }
-        if (full) return;  // table full: the rest waits for the next round
-        bool any = false;
-#pragma unroll
-        for (int u = 0; u < KV; u++) {
-            if (hv.pos[u] + 32 < hv.end[u]) {  // next group of a long row
-                hv.pos[u] += 32;
-                hv.pend[u] = group_mask(hv.end[u] - hv.pos[u]);
-            } else {
-                hv.pos[u] = hv.end[u];
-            }
-            any = any || hv.pend[u] != 0u;
-        }
-        if (!any) hv.held = false;
+        hv.held = false;  // every vertex of the batch is folded
     }
 }
 
-// Phase B: relax every table entry once per slot.  Wave-uniform trip count.
-// The degree of a target below the source is loaded with its label, whether
-// or not the relaxation turns out to be a first reach, so each batch of KR
-// entries costs one memory round trip.
+// Phase B: relax every table entry once per slot.  Each half-wave takes 32
+// entries: lane k loads entry k's CSR range and next-level mask, then the
+// half issues all 32 label-line loads before using any, so a batch costs one
+// memory round trip.  Wave-uniform control flow.
 __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_src,
                             uint32_t* fb, uint32_t& fcount, TallyB& t) {
     const BlkParams& p = *B.p;
     const uint32_t lane = __lane_id();
     const uint32_t half = lane >> 5;
     const uint32_t j = lane & 31u;
-    const bool leader = j == 0;
     const uint32_t w = threadIdx.x / WAVE;
     uint32_t* maskn = B.mask0 + (uint64_t)nbuf * B.W;
-    uint32_t* qn = B.q0 + (uint64_t)nbuf * B.W;
+    uint4* qn = B.q0 + (uint64_t)nbuf * B.W;
     const uint32_t tc = S.tcnt;
     const uint32_t ntab = tc < (uint32_t)T ? tc : (uint32_t)T;
     const uint32_t off = B.off;
-    for (uint32_t base = w * 2 * KR; base < ntab; base += BW * 2 * KR) {
-        uint32_t mm[KR], c[KR], lm[KR], nmk[KR];
-        int32_t d0[KR], d1[KR];
-        bool ok[KR], act[KR], up[KR];
+    for (uint32_t base = w * (2 * KR); base < ntab; base += BW * (2 * KR)) {
+        const uint32_t eb = base + half * KR;
+        // lane k: entry eb + k
+        const uint32_t ek = eb + j;
+        uint32_t mk = HOLE;
+        if (j < (uint32_t)KR && ek < ntab) mk = S.ent_m[ek];
+        const bool okk = mk != HOLE;
+        int32_t d0 = 0, d1 = 0;
+        uint32_t nm = 0;
+        if (okk) {
+            const uint32_t ro = (B.wb + mk) << 2;
+            d0 = at(p.rp, ro);
+            d1 = at(p.rp, ro + 4u);
+            nm = at(maskn, mk << 2);
+        }
+        const uint32_t dd = (uint32_t)(d1 - d0);
+        // lane j: slot j's label of every entry
+        uint32_t lm[KR];
 #pragma unroll
         for (int u = 0; u < KR; u++) {
-            const uint32_t e = base + half * KR + u;
-            ok[u] = e < ntab;
-            mm[u] = ok[u] ? S.ent_m[e] : HOLE;
-            ok[u] = ok[u] && mm[u] != HOLE;
-            c[u] = INF;
-            if (ok[u]) {
-                c[u] = S.cell[e][j];
-                S.cell[e][j] = INF;  // ready for the next round
+            const uint32_t mu = from_lane(mk, u, half);
+            lm[u] = 0u;
+            if (mu != HOLE) lm[u] = at(B.lab0, (mu * SL + j) << 2);
+        }
+        uint32_t mybits = 0;  // lane k: slots that push entry k's target
+#pragma unroll
+        for (int u = 0; u < KR; u++) {
+            const uint32_t mu = from_lane(mk, u, half);
+            uint32_t c = INF;
+            if (mu != HOLE) {
+                c = S.cell[eb + u][j];
+                S.cell[eb + u][j] = INF;  // ready for the next round
             }
-            act[u] = c[u] != INF;
-            up[u] = mm[u] > my_src;
-        }
-        // the one round trip: label, degree, next-level slot mask
-        uint32_t o[KR];
-#pragma unroll
-        for (int u = 0; u < KR; u++) {
-            const bool lo = act[u] && !up[u];
-            o[u] = (mm[u] * SL + j) << 2;
-            const uint32_t ro = (B.wb + mm[u]) << 2;
-            lm[u] = act[u] ? at(B.lab0, o[u]) : 0u;
-            d0[u] = lo ? at(p.rp, ro) : 0;
-            d1[u] = lo ? at(p.rp, ro + 4u) : 0;
-            nmk[u] = (ok[u] && leader) ? at(maskn, mm[u] << 2) : 0u;
-        }
-        bool push[KR], fill[KR];
-#pragma unroll
-        for (int u = 0; u < KR; u++) {
-            push[u] = fill[u] = false;
-            if (act[u]) {
-                if (up[u]) {  // m above the source: U entry on first touch (symbfact.cuh:992-1009)
+            bool push = false, fill = false;
+            if (c != INF) {
+                const uint32_t o = (mu * SL + j) << 2;
+                if (mu > my_src) {  // m above the source: U entry on first touch (symbfact.cuh:992-1009)
                     if (lm[u] >= B.stale) {
-                        at(B.lab0, o[u]) = c[u];
-                        fill[u] = true;
+                        at(B.lab0, o) = c;
+                        fill = true;
                     }
-                } else if (lm[u] > c[u]) {  // relaxation (symbfact.cuh:1018-1063)
-                    at(B.lab0, o[u]) = c[u];
-                    push[u] = mm[u] < my_src;
+                } else if (lm[u] > c) {  // relaxation (symbfact.cuh:1018-1063)
+                    at(B.lab0, o) = c;
+                    push = mu < my_src;
                     if (lm[u] >= B.stale) {  // first reach: |R_<(s)| and TE_alg
                         t.reach++;
-                        t.te += (uint32_t)(d1[u] - d0[u]);
+                        t.te += from_lane(dd, u, half);
                     }
-                    const uint32_t mo = mm[u] + off;
-                    fill[u] = lm[u] >= mo && c[u] < mo;  // label crosses below m: new L entry
+                    const uint32_t mo = mu + off;
+                    fill = lm[u] >= mo && c < mo;  // label crosses below m: new L entry
                 }
             }
+            const uint32_t pb = (uint32_t)(__ballot(push) >> (32u * half));
+            if (j == (uint32_t)u) mybits = pb;
+            if (fill) t.fills++;
+            stage_fill(p, S, fb, fcount, fill, (j << 25) | (mu + B.wb));
         }
-        // next-level slot masks: the half leader writes maskn[m] (m is unique
-        // in the round, so no other lane touches it)
-        uint32_t bits[KR];
-#pragma unroll
-        for (int u = 0; u < KR; u++) bits[u] = (uint32_t)(__ballot(push[u]) >> (32u * half));
-        if (leader) {
-#pragma unroll
-            for (int u = 0; u < KR; u++) {
-                if (bits[u] && (nmk[u] | bits[u]) != nmk[u]) {
-                    at(maskn, mm[u] << 2) = nmk[u] | bits[u];
-                    if (nmk[u] == 0u) qn[atomicAdd(&S.qcnt[nbuf], 1u)] = mm[u];
-                }
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < KR; u++) {
-            if (fill[u]) t.fills++;
-            stage_fill(p, S, fb, fcount, fill[u], (j << 25) | (mm[u] + B.wb));
+        // next-level slot masks and queue entries, one lane per entry (each
+        // target is unique in the round, so no other lane touches its mask)
+        const bool upd = okk && mybits && (nm | mybits) != nm;
+        if (upd) at(maskn, mk << 2) = nm | mybits;
+        const bool enq = upd && nm == 0u;
+        const uint64_t em = __ballot(enq);
+        if (em) {
+            uint32_t qb = 0;
+            if (lane == 0) qb = atomicAdd(&S.qcnt[nbuf], (uint32_t)__popcll(em));
+            qb = bcast0(qb);
+            if (enq) qn[qb + rank_in(em)] = make_uint4(mk, (uint32_t)d0, (uint32_t)d1, 0u);
         }
     }
 }
 
-__global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
+__global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_traverse_blk(BlkParams p) {
     __shared__ BlkLds S;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
@@ -456,13 +496,15 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
                     const uint32_t old = at(B.lab0, o);
                     at(B.lab0, o) = off;
                     if (m < src) {
+                        const int32_t r0 = p.rp[mg], r1 = p.rp[mg + 1];
                         if (old >= B.stale) {
                             rc++;
-                            te += (uint32_t)(p.rp[mg + 1] - p.rp[mg]);
+                            te += (uint32_t)(r1 - r0);
                         }
                         const uint32_t mo = B.mask0[m];  // level-0 buffer
                         B.mask0[m] = mo | (1u << k);
-                        if (mo == 0u) B.q0[atomicAdd(&S.qcnt[0], 1u)] = m;
+                        if (mo == 0u)
+                            B.q0[atomicAdd(&S.qcnt[0], 1u)] = make_uint4(m, (uint32_t)r0, (uint32_t)r1, 0u);
                     }
                 }
                 te = wave_sum64(te);
@@ -484,6 +526,9 @@ __global__ __launch_bounds__(BB) void k_traverse_blk(BlkParams p) {
             const uint32_t nbuf = cur ^ 1u;
             HalfVtx hv;
             hv.held = false;
+            hv.done = false;
+            hv.pend = 0u;
+            hv.pos = hv.end = 0;
             for (;;) {  // rounds
                 combine_phase(B, S, hv, cur, size, t);
                 if (hv.held && j == 0) atomicOr(&S.held, 1u);

@@ -59,7 +59,7 @@ struct BlkParams {
     uint32_t nblocks;
     uint32_t* lab;         // nblocks * W * kBlockSlots labels, [vertex][slot]
     uint32_t* mask;        // nblocks * 2 * W frontier slot masks
-    uint32_t* qg;          // nblocks * 2 * W frontier queue entries
+    uint4* qg;             // nblocks * 2 * W frontier entries (v, row begin, row end, -)
     uint64_t* fills;
     uint64_t fill_cap;
     Ctrl* ctrl;
