@@ -108,9 +108,6 @@ const void* traverse_kernel_ptr();
 size_t traverse_lds_bytes();
 
 hipError_t launch_fill(uint32_t* p, uint64_t count, uint32_t value, hipStream_t s);
-hipError_t launch_pack_rows(const int32_t* colbeg, const int32_t* colend, const int32_t* csr,
-                            const int32_t* rp, int32_t* ci, int32_t n, hipStream_t s);
-
 struct FinalParams {
     int32_t n;
     const int32_t* rp;

@@ -1,6 +1,6 @@
 // gsofa_struct.hip -- device kernels that turn the traversal's fill stream into
 // sorted per-row L and U structure, plus the supernode pass and small helpers
-// (scans, fills, CSR repacking).
+// (scans, fills).
 //
 // The reference never materialises L/U (its markers are transient,
 // with_space_optimization/symbfact.cuh:2126) and has only a stub for
@@ -25,17 +25,6 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
         p[i] = v;
 }
 
-// Repack rows so that row i occupies ci[rp[i] .. rp[i+1]) (rp precomputed).
-__global__ void k_pack_rows(const int32_t* colbeg, const int32_t* colend, const int32_t* csr,
-                            const int32_t* rp, int32_t* ci, int32_t n) {
-    const int lane = threadIdx.x & 63;
-    const int64_t w = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int64_t nw = ((int64_t)gridDim.x * blockDim.x) >> 6;
-    for (int64_t i = w; i < n; i += nw) {
-        const int32_t b = colbeg[i], e = colend[i], o = rp[i];
-        for (int32_t k = b + lane; k < e; k += 64) ci[o + (k - b)] = csr[k];
-    }
-}
 
 __device__ __forceinline__ bool row_processed(int32_t s, int32_t n_gpu, int32_t gpu_id) {
     return gpu_id < 0 || (s % n_gpu) == gpu_id;
@@ -338,14 +327,6 @@ inline dim3 grid_for(int64_t work, int per = SB, int cap = 65536) {
 hipError_t launch_fill(uint32_t* p, uint64_t count, uint32_t value, hipStream_t s) {
     if (count == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_u32, grid_for((int64_t)count), dim3(SB), 0, s, p, count, value);
-    return hipGetLastError();
-}
-
-hipError_t launch_pack_rows(const int32_t* colbeg, const int32_t* colend, const int32_t* csr,
-                            const int32_t* rp, int32_t* ci, int32_t n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_pack_rows, grid_for((int64_t)n * 64), dim3(SB), 0, s, colbeg, colend, csr,
-                       rp, ci, n);
     return hipGetLastError();
 }
 
