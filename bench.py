@@ -81,6 +81,56 @@ def copy_probe(nbytes=1 << 30, reps=3):
           flush=True)
 
 
+def batch_bench(steps, warmup):
+    """C4: 256 ND-ordered 2D grids (side log-uniform in [16,256]).
+
+    batched:    one plan over all 256 matrices, one persistent launch per step
+    one_by_one: the plain path, one plan and one launch per matrix; each plan is
+                created and warmed untimed, then one execute is timed
+    Both run with the matrices resident in HBM; results stay on the device.
+    """
+    import numpy as np
+
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    mats = cf.config4()
+    with g.Plan(mats) as p:
+        for _ in range(warmup):
+            p.execute()
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            p.execute()
+            times.append(time.perf_counter() - t0)
+        fills = [p.fetch_matrix(b).fill_count for b in range(len(mats))]
+    batched_s = sum(times) / len(times)
+    lat = []
+    for A in mats:
+        with g.Plan(A) as q:
+            q.execute()
+            t0 = time.perf_counter()
+            q.execute()
+            lat.append(time.perf_counter() - t0)
+    obo_s = sum(lat)
+    gold_ok = None
+    gpath = os.path.join(ROOT, "tests", "golden", "goldens.json")
+    if os.path.exists(gpath):
+        gold = json.load(open(gpath)).get("c4")
+        if gold:
+            gold_ok = [x["fill_count"] for x in gold] == fills
+    lat_ms = np.array(lat) * 1e3
+    return {"workload": "256 ND 2D grids, side log-uniform [16,256], seed 4",
+            "matrices": len(mats), "vertices": int(sum(A.n for A in mats)),
+            "batched_ms": round(batched_s * 1e3, 3),
+            "batched_matrices_per_s": round(len(mats) / batched_s, 1),
+            "one_by_one_ms": round(obo_s * 1e3, 3),
+            "one_by_one_matrices_per_s": round(len(mats) / obo_s, 1),
+            "speedup_vs_one_by_one": round(obo_s / batched_s, 2),
+            "one_by_one_latency_ms_p50": round(float(np.percentile(lat_ms, 50)), 3),
+            "one_by_one_latency_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
+            "matches_golden": gold_ok}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +142,7 @@ def main():
     ap.add_argument("--grid-blocks", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-matrix batch leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -162,6 +213,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic},
     }
+    if not args.no_batch:
+        out["batch"] = batch_bench(args.steps, args.warmup)
+        log("batch:", json.dumps(out["batch"]))
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(A, args.cpu_seconds, threads)

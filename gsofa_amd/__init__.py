@@ -9,7 +9,10 @@ reference's operator surface for this path:
   the sorted L/U structure and the supernode partition;
 * ``read_ascii`` / ``write_ascii`` handle the reference's three-file format
   (with_space_optimization/symbfact.cuh:1591-1640);
-* ``Plan`` keeps a matrix resident on the device for repeated execution.
+* ``Plan`` keeps a matrix (or a batch of matrices) resident on the device for
+  repeated execution;
+* ``Server`` admits requests as they arrive and runs them in continuously
+  formed batches (no counterpart in the reference).
 
 Errors surface as ``GsofaError`` carrying the library status code.
 """
@@ -20,7 +23,7 @@ import numpy as np
 from . import _lib
 from ._lib import GSOFA_FLAG_COUNTS_ONLY
 
-__all__ = ["CsrMatrix", "SymbfactResult", "GsofaError", "Plan", "symbfact", "gen_grid",
+__all__ = ["CsrMatrix", "SymbfactResult", "GsofaError", "Plan", "Server", "symbfact", "gen_grid",
            "gen_band", "read_ascii", "write_ascii", "footprint", "GSOFA_FLAG_COUNTS_ONLY"]
 
 
@@ -240,6 +243,72 @@ class Plan:
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().gsofa_plan_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Request server with continuous batching (include/gsofa.h, gsofa_server_*).
+
+    submit(A) copies the matrix and returns a ticket; wait(ticket) / poll(ticket)
+    return (SymbfactResult, info) once the request is final, where info holds
+    queued_ms, latency_ms, batch_ms, batch_id and batch_size.
+    """
+
+    def __init__(self, device=0, mem_gb=0.0, max_inflight=2, max_batch=256, max_supernode_size=0,
+                 counts_only=False):
+        o = _lib.ServerOptions()
+        _lib.lib().gsofa_server_options_default(C.byref(o))
+        o.device = device
+        o.mem_budget_bytes = int(mem_gb * (1 << 30))
+        o.max_inflight = max_inflight
+        o.max_batch = max_batch
+        o.max_supernode_size = max_supernode_size
+        o.flags = GSOFA_FLAG_COUNTS_ONLY if counts_only else 0
+        h = C.c_void_p()
+        _check(_lib.lib().gsofa_server_create(C.byref(o), C.byref(h)), "server_create")
+        self._h = h
+
+    def submit(self, A):
+        m = A.to_c()
+        t = C.c_int64()
+        _check(_lib.lib().gsofa_server_submit(self._h, C.byref(m), C.byref(t)), "server_submit")
+        return int(t.value)
+
+    def _final(self, rc, r, info):
+        _check(rc, "server request")
+        try:
+            res = SymbfactResult(r)
+        finally:
+            _lib.lib().gsofa_result_free(C.byref(r))
+        return res, {"queued_ms": info.queued_ms, "latency_ms": info.latency_ms,
+                     "batch_ms": info.batch_ms, "batch_id": info.batch_id,
+                     "batch_size": info.batch_size}
+
+    def poll(self, ticket):
+        """None while the request runs, else (result, info)."""
+        r, info = _lib.Result(), _lib.RequestInfo()
+        rc = _lib.lib().gsofa_server_poll(self._h, ticket, C.byref(r), C.byref(info))
+        if rc == _lib.GSOFA_PENDING:
+            return None
+        return self._final(rc, r, info)
+
+    def wait(self, ticket):
+        r, info = _lib.Result(), _lib.RequestInfo()
+        rc = _lib.lib().gsofa_server_wait(self._h, ticket, C.byref(r), C.byref(info))
+        return self._final(rc, r, info)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().gsofa_server_destroy(self._h)
             self._h = None
 
     def __enter__(self):

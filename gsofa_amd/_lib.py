@@ -20,7 +20,10 @@ EXPORTS = [
     "gsofa_plan_destroy", "gsofa_plan_footprint", "gsofa_read_ascii", "gsofa_write_ascii",
     "gsofa_gen_grid", "gsofa_gen_band", "gsofa_gen_side_loguniform", "gsofa_gen_uniform_int",
     "gsofa_gen_unit", "gsofa_batch_create", "gsofa_plan_matrix_count", "gsofa_plan_fetch_matrix",
+    "gsofa_server_options_default", "gsofa_server_create", "gsofa_server_submit",
+    "gsofa_server_poll", "gsofa_server_wait", "gsofa_server_destroy",
 ]
+GSOFA_PENDING = 1
 
 
 class Matrix(C.Structure):
@@ -52,6 +55,16 @@ class Result(C.Structure):
                 ("grid_blocks", C.c_int32),
                 ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64),
                 ("row_te", C.c_void_p), ("row_reach", C.c_void_p)]
+
+
+class ServerOptions(C.Structure):
+    _fields_ = [("device", C.c_int32), ("mem_budget_bytes", C.c_int64), ("max_inflight", C.c_int32),
+                ("max_batch", C.c_int32), ("max_supernode_size", C.c_int32), ("flags", C.c_int32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("queued_ms", C.c_double), ("latency_ms", C.c_double), ("batch_ms", C.c_double),
+                ("batch_id", C.c_int32), ("batch_size", C.c_int32)]
 
 
 _lib = None
@@ -104,5 +117,15 @@ def lib():
     L.gsofa_gen_uniform_int.restype = C.c_int32
     L.gsofa_gen_unit.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64]
     L.gsofa_gen_unit.restype = C.c_double
+    L.gsofa_server_options_default.argtypes = [P(ServerOptions)]
+    L.gsofa_server_create.argtypes = [P(ServerOptions), P(C.c_void_p)]
+    L.gsofa_server_create.restype = C.c_int
+    L.gsofa_server_submit.argtypes = [C.c_void_p, P(Matrix), P(C.c_int64)]
+    L.gsofa_server_submit.restype = C.c_int
+    L.gsofa_server_poll.argtypes = [C.c_void_p, C.c_int64, P(Result), P(RequestInfo)]
+    L.gsofa_server_poll.restype = C.c_int
+    L.gsofa_server_wait.argtypes = [C.c_void_p, C.c_int64, P(Result), P(RequestInfo)]
+    L.gsofa_server_wait.restype = C.c_int
+    L.gsofa_server_destroy.argtypes = [C.c_void_p]
     _lib = L
     return L

@@ -32,7 +32,6 @@ struct gsofa_plan {
     int mode = GSOFA_MODE_BLOCK;
     int32_t C = 0;        // grid mode: sources per group; block mode: nblocks * S
     uint32_t S = 0;       // block mode: slots per workgroup
-    uint32_t ql_cap = 0, qg_cap = 0;
     size_t blk_lds = 0;
     uint32_t* d_mask = nullptr;
     uint4* d_qgv = nullptr;
@@ -264,13 +263,6 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
     }
 }
 
-// Block mode: as many workgroups as can be co-resident (or as the chunk
-// option asks: chunk = concurrent sources), shrunk until the per-workgroup
-// arrays (labels [W][32] + two masks + two 16-byte-entry queues = 168 W
-// bytes) fit the budget.
-int64_t block_bytes(uint32_t nb, uint32_t W) {
-    return (int64_t)nb * ((4ll * kBlockSlots + 8 + 32) * W + 8);
-}
 
 // Work table: chunks of up to kBlockSlots sources of one matrix, in the order
 // they are handed out (decreasing global source).  Shares of a single matrix
@@ -307,47 +299,28 @@ void build_chunks(gsofa_plan* P) {
             P->h_chunks.push_back(ch);
         }
     }
+    // Longest first: a source's traversal runs through the vertices numbered
+    // below it, so its local index bounds its reach; the deepest chunks of
+    // every matrix start before the short ones fill in behind them.
+    std::stable_sort(P->h_chunks.begin(), P->h_chunks.end(), [&](const BlkChunk& a, const BlkChunk& b) {
+        return a.first - P->h_mat_base[a.mat] > b.first - P->h_mat_base[b.mat];
+    });
     P->share_chunk_off[1] = (uint32_t)P->h_chunks.size();
 }
 
-int size_block_mode(gsofa_plan* P, const hipDeviceProp_t& prop, int64_t avail) {
-    const int32_t n = P->n;
-    if (P->opt.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, P->opt.mem_budget_bytes);
-    hipFuncAttributes fa;
-    if (hipFuncGetAttributes(&fa, blk_kernel_ptr()) != hipSuccess) {
-        (void)hipGetLastError();
-        return GSOFA_ERR_HIP;
-    }
-    int max_blk = 0;
-    if (hipDeviceGetAttribute(&max_blk, hipDeviceAttributeMaxSharedMemoryPerBlock, P->device) != hipSuccess)
-        max_blk = 65536;
-    const int64_t per_cu = prop.maxSharedMemoryPerMultiProcessor > 0 ? (int64_t)prop.maxSharedMemoryPerMultiProcessor
-                                                                     : 65536;
-    (void)max_blk;
-    (void)per_cu;
-    P->ql_cap = 0;
-    P->blk_lds = 0;  // all LDS of the block kernel is static
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(), blk_block_size(), 0) !=
-            hipSuccess ||
-        occ <= 0) {
-        (void)hipGetLastError();
-        return GSOFA_ERR_HIP;
-    }
-    uint32_t nb = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
-    if (P->opt.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)P->opt.grid_blocks);
-    if (P->opt.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, P->opt.chunk / kBlockSlots));
+int size_block_mode(gsofa_plan* P, int64_t avail) {
     const int32_t N = std::max(1, P->opt.n_virtual_gpu);
-    const uint64_t maxsrc = std::max<uint64_t>(share_count(n, N, 0), 1);
+    const uint64_t maxsrc = std::max<uint64_t>(share_count(P->n, N, 0), 1);
     const uint64_t nchunks = P->n_mats == 1 ? (maxsrc + kBlockSlots - 1) / kBlockSlots
                                             : std::max<uint64_t>(P->h_chunks.size(), 1);
-    nb = (uint32_t)std::min<uint64_t>(nb, nchunks);
-    while (block_bytes(nb, P->W) > avail && nb > 1) nb = nb > 64 ? nb - nb / 8 : nb - 1;
-    if (block_bytes(nb, P->W) > avail) return GSOFA_ERR_BUDGET;
+    uint32_t nb = 0;
+    const int rc = gsofa::block_grid(P->opt, nchunks, P->W, avail, &nb);
+    if (rc) return rc;
     P->nblocks = nb;
     P->S = kBlockSlots;
     P->C = (int32_t)(nb * kBlockSlots);
     P->Q = 0;
+    P->blk_lds = 0;  // all LDS of the block kernel is static
     return GSOFA_OK;
 }
 
@@ -436,6 +409,49 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 
 }  // namespace
 
+namespace gsofa {
+
+// Block mode: per workgroup, labels [W][32] + two masks + two 16-byte-entry
+// queues = 168 W bytes (+ 8 of state).
+int64_t block_bytes(uint32_t nb, uint32_t W) {
+    return (int64_t)nb * ((4ll * kBlockSlots + 8 + 32) * W + 8);
+}
+
+// Everything else a block-mode plan allocates: CSR, fill stream, per-row
+// arrays, scan scratch, work table, and the L/U columns (+ sort scratch)
+// reserved for nnz + fill_cap entries.
+int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap) {
+    return 4 * (n + 1) + 4 * nnz + 8 * (int64_t)fill_cap + 96 * n + 8 * (int64_t)scan_tmp_words(n) +
+           16 * (int64_t)nchunks + 4 * ((int64_t)n_mats + 1) + 8 * (nnz + (int64_t)fill_cap) + (1 << 20);
+}
+
+uint64_t default_fill_cap(int64_t nnz) { return std::max<uint64_t>(64 * (uint64_t)nnz, 1 << 20); }
+
+// As many workgroups as can be co-resident (or as the options ask: chunk =
+// concurrent sources, grid_blocks), at most one per chunk, shrunk until the
+// per-workgroup arrays fit `avail` bytes (and opt.mem_budget_bytes if set).
+int block_grid(const gsofa_options& o, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out) {
+    if (o.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, o.mem_budget_bytes);
+    int occ = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(), blk_block_size(), 0) !=
+            hipSuccess ||
+        occ <= 0 ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
+    uint32_t nb = (uint32_t)occ * (uint32_t)cus;
+    if (o.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)o.grid_blocks);
+    if (o.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, o.chunk / kBlockSlots));
+    nb = (uint32_t)std::min<uint64_t>(nb, std::max<uint64_t>(nchunks, 1));
+    while (block_bytes(nb, W) > avail && nb > 1) nb = nb > 64 ? nb - nb / 8 : nb - 1;
+    if (block_bytes(nb, W) > avail) return GSOFA_ERR_BUDGET;
+    *nb_out = nb;
+    return GSOFA_OK;
+}
+
+}  // namespace gsofa
+
 extern "C" {
 
 const char* gsofa_strerror(int status) {
@@ -471,9 +487,17 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     if (opt) o = *opt;
     else gsofa_options_default(&o);
     if (!A || A->n <= 0) return 0;
-    Sizing sz;
-    if (choose_sizing(A, &o, A->nnz, INT64_MAX / 4, &sz) != GSOFA_OK) return -1;
-    return sz.total_bytes;
+    if (o.mode == GSOFA_MODE_GRID) {
+        Sizing sz;
+        if (choose_sizing(A, &o, A->nnz, INT64_MAX / 4, &sz) != GSOFA_OK) return -1;
+        return sz.total_bytes;
+    }
+    const int32_t N = std::max(1, o.n_virtual_gpu);
+    const uint64_t nchunks = (share_count(A->n, N, 0) + kBlockSlots - 1) / kBlockSlots;
+    const uint64_t fc = o.output_capacity > 0 ? (uint64_t)o.output_capacity : default_fill_cap(A->nnz);
+    uint32_t nb = 0;
+    if (block_grid(o, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
+    return block_bytes(nb, (uint32_t)A->n) + block_fixed_bytes(A->n, A->nnz, 1, nchunks, fc);
 }
 
 }  // extern "C"
@@ -543,7 +567,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     int e = 0;
     if (P->mode == GSOFA_MODE_BLOCK) {
         build_chunks(P);
-        rc = size_block_mode(P, prop, std::max<int64_t>(avail, 1 << 20));
+        rc = size_block_mode(P, std::max<int64_t>(avail, 1 << 20));
         if (rc) {
             (void)hipStreamDestroy(P->stream);
             delete P;
@@ -576,8 +600,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         e |= dalloc(&P->d_q0, (size_t)P->Q);
         e |= dalloc(&P->d_q1, (size_t)P->Q);
     }
-    P->fill_cap = P->opt.output_capacity > 0 ? (uint64_t)P->opt.output_capacity
-                                             : std::max<uint64_t>(64 * (uint64_t)P->nnz, 1 << 20);
+    P->fill_cap = P->opt.output_capacity > 0 ? (uint64_t)P->opt.output_capacity : default_fill_cap(P->nnz);
     e |= dalloc(&P->d_rp, (size_t)n + 1);
     e |= dalloc(&P->d_ci, (size_t)P->nnz);
     e |= dalloc(&P->d_fills, (size_t)P->fill_cap);

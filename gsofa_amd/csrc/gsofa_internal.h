@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "gsofa.h"
+
 namespace gsofa {
 
 // Device control block.  Zeroed by the host before every traversal launch.
@@ -68,6 +70,11 @@ struct BlkParams {
     unsigned long long* row_reach;  // n: |R_<(s)| of each processed source row
 };
 int blk_block_size();
+// Block-mode sizing shared by plan creation, footprints and the server.
+int64_t block_bytes(uint32_t nb, uint32_t W);
+int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap);
+uint64_t default_fill_cap(int64_t nnz);
+int block_grid(const gsofa_options& o, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
 const void* blk_kernel_ptr();
 hipError_t launch_traverse_blk(const BlkParams& p, size_t lds_bytes, hipStream_t s);
 

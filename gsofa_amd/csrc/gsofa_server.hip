@@ -1,0 +1,298 @@
+// gsofa_server.hip -- request server with continuous batching (host code).
+//
+// No counterpart in the reference, whose driver runs one matrix per process
+// (with_space_optimization/symbfact.cuh:1553).  Requests arrive one at a time
+// from any host thread.  max_inflight worker threads each own one plan at a
+// time (and so one HIP stream): a free worker admits the waiting requests,
+// oldest first, while the batch fits its share of the device budget, runs
+// them as one batch plan (one persistent traversal launch; every matrix keeps
+// its own result) and publishes the results.  Requests that arrive while
+// batches run wait for the next free worker, so batches of different workers
+// overlap on the device and the queue drains continuously.
+//
+// Admission: each worker owns budget / max_inflight bytes.  A batch's bytes
+// are its fixed arrays (CSR, fill stream, outputs; block_fixed_bytes) plus its
+// traversal arrays, which take whatever the fixed part leaves of the share
+// (the plan shrinks its grid to fit, block_grid).  A request is added while at
+// least one workgroup still fits; one that cannot fit an otherwise empty share
+// fails with GSOFA_ERR_BUDGET.
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "gsofa.h"
+#include "gsofa_internal.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+struct Request {
+    int64_t ticket = 0;
+    std::vector<int32_t> colbeg, colend, csr;  // owned copy, rows as submitted
+    int32_t n = 0;
+    int64_t nnz = 0;
+    Clock::time_point t_submit, t_start, t_done;
+    bool done = false;
+    int status = GSOFA_PENDING;
+    gsofa_result res{};
+    gsofa_request_info info{};
+};
+
+}  // namespace
+
+struct gsofa_server {
+    gsofa_server_options opt{};
+    int64_t share = 0;  // device bytes per worker
+    std::mutex mu;
+    std::condition_variable cv_work;  // waiting requests or shutdown
+    std::condition_variable cv_done;  // a request became final
+    std::deque<Request*> waiting;
+    std::unordered_map<int64_t, Request*> live;  // every request not yet taken
+    int64_t next_ticket = 1;
+    int32_t next_batch = 1;
+    bool stop = false;
+    std::vector<std::thread> workers;
+};
+
+namespace {
+
+// Bytes the batch `reqs` would need apart from its traversal arrays.
+int64_t fixed_bytes(const std::vector<Request*>& reqs) {
+    int64_t n = 0, nnz = 0;
+    uint64_t nchunks = 0;
+    for (const Request* r : reqs) {
+        n += r->n;
+        nnz += r->nnz;
+        nchunks += ((uint64_t)r->n + gsofa::kBlockSlots - 1) / gsofa::kBlockSlots;
+    }
+    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, gsofa::default_fill_cap(nnz));
+}
+
+uint32_t largest(const std::vector<Request*>& reqs) {
+    uint32_t W = 1;
+    for (const Request* r : reqs) W = std::max<uint32_t>(W, (uint32_t)r->n);
+    return W;
+}
+
+// Traversal bytes left for the batch within one share (<= 0: does not fit).
+int64_t trav_room(const gsofa_server* S, const std::vector<Request*>& reqs) {
+    return S->share - fixed_bytes(reqs) - gsofa::block_bytes(1, largest(reqs));
+}
+
+void finish(gsofa_server* S, Request* r, int status) {
+    r->status = status;
+    r->done = true;
+    r->t_done = Clock::now();
+    r->info.latency_ms = ms_between(r->t_submit, r->t_done);
+    r->info.queued_ms = ms_between(r->t_submit, r->t_start);
+}
+
+void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id) {
+    std::vector<gsofa_matrix> mats(batch.size());
+    for (size_t i = 0; i < batch.size(); i++)
+        mats[i] = {batch[i]->n, batch[i]->nnz, batch[i]->colbeg.data(), batch[i]->colend.data(),
+                   batch[i]->csr.data()};
+    gsofa_options o;
+    gsofa_options_default(&o);
+    o.device = S->opt.device;
+    o.max_supernode_size = S->opt.max_supernode_size;
+    o.flags = S->opt.flags;
+    // the traversal arrays get what the fixed part leaves of the share
+    o.mem_budget_bytes = std::max<int64_t>(S->share - fixed_bytes(batch), 1);
+    gsofa_plan* P = nullptr;
+    int rc = gsofa_batch_create(mats.data(), (int32_t)mats.size(), &o, &P);
+    if (rc == GSOFA_OK) rc = gsofa_plan_execute(P);
+    std::vector<int> st(batch.size(), rc);
+    std::vector<gsofa_result> res(batch.size());
+    double batch_ms = 0;
+    if (rc == GSOFA_OK) {
+        for (size_t i = 0; i < batch.size(); i++) {
+            st[i] = gsofa_plan_fetch_matrix(P, (int32_t)i, &res[i]);
+            if (st[i] == GSOFA_OK) batch_ms = res[i].ms;
+        }
+    }
+    if (P) gsofa_plan_destroy(P);
+    std::lock_guard<std::mutex> lk(S->mu);
+    for (size_t i = 0; i < batch.size(); i++) {
+        Request* r = batch[i];
+        if (st[i] == GSOFA_OK) r->res = res[i];
+        r->info.batch_id = batch_id;
+        r->info.batch_size = (int32_t)batch.size();
+        r->info.batch_ms = batch_ms;
+        std::vector<int32_t>().swap(r->colbeg);  // the input copy is no longer needed
+        std::vector<int32_t>().swap(r->colend);
+        std::vector<int32_t>().swap(r->csr);
+        finish(S, r, st[i]);
+    }
+    S->cv_done.notify_all();
+}
+
+void worker_main(gsofa_server* S) {
+    if (hipSetDevice(S->opt.device) != hipSuccess) (void)hipGetLastError();
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        S->cv_work.wait(lk, [&] { return S->stop || !S->waiting.empty(); });
+        if (S->waiting.empty()) break;  // stopping and drained
+        std::vector<Request*> batch;
+        while (!S->waiting.empty() && (int32_t)batch.size() < S->opt.max_batch) {
+            batch.push_back(S->waiting.front());
+            if (trav_room(S, batch) < 0) {
+                batch.pop_back();
+                break;
+            }
+            S->waiting.pop_front();
+        }
+        if (batch.empty()) {  // the oldest request cannot fit a share on its own
+            Request* r = S->waiting.front();
+            S->waiting.pop_front();
+            r->t_start = Clock::now();
+            finish(S, r, GSOFA_ERR_BUDGET);
+            S->cv_done.notify_all();
+            continue;
+        }
+        const int32_t id = S->next_batch++;
+        const Clock::time_point t0 = Clock::now();
+        for (Request* r : batch) r->t_start = t0;
+        if (!S->waiting.empty()) S->cv_work.notify_one();  // another worker may take the rest
+        lk.unlock();
+        run_batch(S, batch, id);
+        lk.lock();
+    }
+}
+
+int take(gsofa_server* S, Request* r, gsofa_result* out, gsofa_request_info* info) {
+    // caller holds S->mu; r is final
+    const int st = r->status;
+    if (info) *info = r->info;
+    if (st == GSOFA_OK && out) *out = r->res;
+    else if (st == GSOFA_OK) gsofa_result_free(&r->res);
+    S->live.erase(r->ticket);
+    delete r;
+    return st;
+}
+
+}  // namespace
+
+extern "C" {
+
+void gsofa_server_options_default(gsofa_server_options* o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->device = 0;
+    o->mem_budget_bytes = 0;
+    o->max_inflight = 2;
+    o->max_batch = 256;
+    o->max_supernode_size = 0;
+    o->flags = 0;
+}
+
+int gsofa_server_create(const gsofa_server_options* opt, gsofa_server** out) {
+    if (!out) return GSOFA_ERR_INVALID;
+    *out = nullptr;
+    gsofa_server_options o;
+    if (opt) o = *opt;
+    else gsofa_server_options_default(&o);
+    if (o.max_inflight < 1 || o.max_inflight > 16 || o.max_batch < 1) return GSOFA_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_NO_DEVICE;
+    }
+    if (o.device < 0 || o.device >= ndev) return GSOFA_ERR_INVALID;
+    if (o.mem_budget_bytes <= 0) {
+        size_t freeb = 0, totalb = 0;
+        if (hipSetDevice(o.device) != hipSuccess || hipMemGetInfo(&freeb, &totalb) != hipSuccess) {
+            (void)hipGetLastError();
+            return GSOFA_ERR_HIP;
+        }
+        o.mem_budget_bytes = (int64_t)((double)freeb * 0.80);
+    }
+    gsofa_server* S = new gsofa_server();
+    S->opt = o;
+    S->share = o.mem_budget_bytes / o.max_inflight;
+    for (int32_t i = 0; i < o.max_inflight; i++) S->workers.emplace_back(worker_main, S);
+    *out = S;
+    return GSOFA_OK;
+}
+
+int gsofa_server_submit(gsofa_server* S, const gsofa_matrix* A, int64_t* ticket) {
+    if (!S || !A || !ticket) return GSOFA_ERR_INVALID;
+    if (A->n < 1 || A->nnz < 0 || (A->nnz > 0 && !A->csr) || !A->colbeg || !A->colend)
+        return GSOFA_ERR_INVALID;
+    Request* r = new Request();
+    r->n = A->n;
+    // pack rows in order so the copy is exactly nnz entries
+    r->colbeg.resize((size_t)A->n);
+    r->colend.resize((size_t)A->n);
+    int64_t k = 0;
+    for (int32_t i = 0; i < A->n; i++) {
+        const int64_t b = A->colbeg[i], e = A->colend[i];
+        if (b < 0 || e < b || e > A->nnz) {
+            delete r;
+            return GSOFA_ERR_INVALID;
+        }
+        r->colbeg[i] = (int32_t)k;
+        r->csr.insert(r->csr.end(), A->csr + b, A->csr + e);
+        k += e - b;
+        r->colend[i] = (int32_t)k;
+    }
+    r->nnz = k;
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->stop) {
+        delete r;
+        return GSOFA_ERR_STATE;
+    }
+    r->ticket = S->next_ticket++;
+    S->live[r->ticket] = r;
+    S->waiting.push_back(r);
+    *ticket = r->ticket;
+    S->cv_work.notify_one();
+    return GSOFA_OK;
+}
+
+int gsofa_server_poll(gsofa_server* S, int64_t ticket, gsofa_result* out, gsofa_request_info* info) {
+    if (!S) return GSOFA_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->live.find(ticket);
+    if (it == S->live.end()) return GSOFA_ERR_INVALID;
+    if (!it->second->done) return GSOFA_PENDING;
+    return take(S, it->second, out, info);
+}
+
+int gsofa_server_wait(gsofa_server* S, int64_t ticket, gsofa_result* out, gsofa_request_info* info) {
+    if (!S) return GSOFA_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->live.find(ticket);
+    if (it == S->live.end()) return GSOFA_ERR_INVALID;
+    Request* r = it->second;
+    S->cv_done.wait(lk, [&] { return r->done; });
+    return take(S, r, out, info);
+}
+
+void gsofa_server_destroy(gsofa_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+    }
+    S->cv_work.notify_all();
+    for (std::thread& t : S->workers) t.join();
+    for (auto& kv : S->live) {
+        if (kv.second->done && kv.second->status == GSOFA_OK) gsofa_result_free(&kv.second->res);
+        delete kv.second;
+    }
+    delete S;
+}
+
+}  // extern "C"

@@ -165,6 +165,53 @@ int32_t gsofa_plan_matrix_count(const gsofa_plan* plan);
 /* Result of matrix `index` of the last execute, in that matrix's own numbering. */
 int gsofa_plan_fetch_matrix(gsofa_plan* plan, int32_t index, gsofa_result* out);
 
+/* ---- request server (continuous batching) -----------------------------------
+ * No counterpart in the reference (one matrix per process run).  Requests are
+ * submitted one at a time from any host thread; the server copies the matrix
+ * and returns a ticket.  max_inflight worker threads each own a HIP stream:
+ * whenever a worker is free it admits the waiting requests, oldest first, up
+ * to max_batch and while the device bytes of all in-flight batches stay within
+ * mem_budget_bytes, runs them as one batch plan (one persistent launch) and
+ * publishes every request's result.  Batches of different workers overlap on
+ * the device.  Each result equals the request's stand-alone result.
+ * A request whose own footprint exceeds budget / max_inflight fails with
+ * GSOFA_ERR_BUDGET. */
+#define GSOFA_PENDING 1 /* gsofa_server_poll: the request is not finished */
+
+typedef struct gsofa_server gsofa_server;
+
+typedef struct gsofa_server_options {
+    int32_t device;
+    int64_t mem_budget_bytes;   /* device bytes for all in-flight batches; 0 = 80% of free */
+    int32_t max_inflight;       /* batches on the device at once (worker threads) */
+    int32_t max_batch;          /* requests per batch */
+    int32_t max_supernode_size; /* as gsofa_options */
+    int32_t flags;              /* GSOFA_FLAG_* */
+} gsofa_server_options;
+
+typedef struct gsofa_request_info {
+    double queued_ms;   /* submit -> its batch starts */
+    double latency_ms;  /* submit -> result published */
+    double batch_ms;    /* device time of its batch (traversal + structure + supernodes) */
+    int32_t batch_id;   /* 1, 2, ... in admission order */
+    int32_t batch_size; /* requests in its batch */
+} gsofa_request_info;
+
+void gsofa_server_options_default(gsofa_server_options* opt);
+int gsofa_server_create(const gsofa_server_options* opt, gsofa_server** server);
+/* Copies A; *ticket identifies the request until its result is taken. */
+int gsofa_server_submit(gsofa_server* server, const gsofa_matrix* A, int64_t* ticket);
+/* GSOFA_PENDING while running; otherwise the request's final status, with its
+ * result moved to *out (on GSOFA_OK) and the ticket released. */
+int gsofa_server_poll(gsofa_server* server, int64_t ticket, gsofa_result* out,
+                      gsofa_request_info* info);
+/* Blocks until the request is final, then behaves as gsofa_server_poll. */
+int gsofa_server_wait(gsofa_server* server, int64_t ticket, gsofa_result* out,
+                      gsofa_request_info* info);
+/* Finishes every admitted and waiting request, then releases the server;
+ * results not taken are freed. */
+void gsofa_server_destroy(gsofa_server* server);
+
 /* ---- inputs: reference file format and generators ---------------------------
  * ASCII triple (with_space_optimization/symbfact.cuh:1591-1640): the csr file
  * holds nnz, n and nnz indices; colbeg/colend hold n integers each. */

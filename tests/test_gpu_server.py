@@ -1,0 +1,129 @@
+"""GPU: the request server (continuous batching) returns, for every request,
+exactly the stand-alone result, whatever batch it lands in."""
+import sys
+import threading
+import time
+
+import numpy as np
+import pytest
+
+from conftest import random_matrix
+from gpu_util import assert_same_structure
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _same(a, b):
+    assert a.fill_count == b.fill_count
+    assert a.te_alg == b.te_alg and a.reach == b.reach
+    for x, y in ((a.L_rowptr, b.L_rowptr), (a.L_col, b.L_col), (a.U_rowptr, b.U_rowptr),
+                 (a.U_col, b.U_col), (a.row_fill, b.row_fill), (a.super_start, b.super_start),
+                 (a.supno, b.supno)):
+        np.testing.assert_array_equal(x, y)
+
+
+def _requests():
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    rng = np.random.default_rng(2024)
+    reqs = []
+    for k in range(24):
+        n = int(rng.integers(1, 400))
+        reqs.append(random_matrix(rng, n, float(rng.choice([0.005, 0.02, 0.06])))[0])
+    for b, s in enumerate([16, 23, 40, 64, 90]):
+        reqs.append(cf.batch_matrix(s, b))
+    reqs.append(g.gen_grid(12, 12, 12, nd_leaf=64, seed=7))
+    reqs.append(g.gen_band(5000, 256, 8.0, 9))
+    return reqs
+
+
+def test_server_mix_equals_alone(bcsstk18):
+    """Requests submitted back to back land in several batches of mixed size;
+    each result equals the same matrix run alone (and the oracle for the
+    small ones)."""
+    import gsofa_amd as g
+    from oracle import oracle
+    reqs = _requests() + [bcsstk18]
+    _say("case: server mix of", len(reqs), "requests, 2 in flight, batches <= 8")
+    with g.Server(max_inflight=2, max_batch=8) as srv:
+        tickets = [srv.submit(A) for A in reqs]
+        out = [srv.wait(t) for t in tickets]
+    batches = {info["batch_id"] for _, info in out}
+    assert len(batches) >= 4
+    assert max(info["batch_size"] for _, info in out) > 1
+    for i, (A, (r, info)) in enumerate(zip(reqs, out)):
+        _say("check request", i, "n", A.n, "batch", info["batch_id"], "of", info["batch_size"])
+        assert info["latency_ms"] >= info["queued_ms"] >= 0.0
+        _same(r, g.symbfact(A))
+        if A.n <= 2000:
+            assert_same_structure(r, oracle.symbfact(A.colbeg, A.colend, A.csr, A.n))
+
+
+def test_server_concurrent_submitters():
+    """Three host threads submit and wait at the same time."""
+    import gsofa_amd as g
+    reqs = _requests()
+    alone = [g.symbfact(A) for A in reqs]
+    results = [None] * len(reqs)
+    errors = []
+    _say("case: server, 3 submitting threads,", len(reqs), "requests")
+    with g.Server(max_inflight=3, max_batch=6) as srv:
+        def run(idx):
+            try:
+                ts = [(i, srv.submit(reqs[i])) for i in idx]
+                for i, t in ts:
+                    results[i] = srv.wait(t)[0]
+            except Exception as e:  # surfaced below
+                errors.append(e)
+        th = [threading.Thread(target=run, args=(list(range(k, len(reqs), 3)),)) for k in range(3)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+    assert not errors, errors
+    for r, a in zip(results, alone):
+        _same(r, a)
+
+
+def test_server_budget_rejects_oversized():
+    """A request that cannot fit a worker's share on its own fails with
+    GSOFA_ERR_BUDGET; the others complete."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    small = cf.batch_matrix(20, 1)
+    big = g.gen_band(200000, 256, 8.0, 3)
+    _say("case: server budget 0.25 GiB, one oversized request")
+    with g.Server(max_inflight=1, max_batch=4, mem_gb=0.25) as srv:
+        t1 = srv.submit(small)
+        t2 = srv.submit(big)
+        t3 = srv.submit(small)
+        r1, _ = srv.wait(t1)
+        with pytest.raises(g.GsofaError) as ei:
+            srv.wait(t2)
+        assert ei.value.code == -7
+        r3, _ = srv.wait(t3)
+    _same(r1, r3)
+
+
+def test_server_poll_and_unknown_ticket():
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    A = cf.batch_matrix(30, 2)
+    with g.Server() as srv:
+        t = srv.submit(A)
+        got = None
+        for _ in range(2000):  # up to ~10 s of polling, then block
+            got = srv.poll(t)
+            if got is not None:
+                break
+            time.sleep(0.005)
+        if got is None:
+            got = srv.wait(t)
+        _same(got[0], g.symbfact(A))
+        with pytest.raises(g.GsofaError):
+            srv.poll(t)  # taken

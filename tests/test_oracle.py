@@ -324,3 +324,21 @@ def test_invalid_matrix_rejected_before_device():
     with pytest.raises(g.GsofaError) as ei:
         g.symbfact(A)
     assert ei.value.code in (-1, -6)
+
+
+def test_server_argument_checks():
+    """Server creation validates its options before touching a device; without
+    a GPU it reports GSOFA_ERR_NO_DEVICE rather than failing later."""
+    import gsofa_amd as g
+    with pytest.raises(g.GsofaError) as ei:
+        g.Server(max_inflight=0)
+    assert ei.value.code == -1
+    try:
+        import torch
+        has_gpu = torch.cuda.is_available()
+    except ImportError:
+        has_gpu = False
+    if not has_gpu:
+        with pytest.raises(g.GsofaError) as ei:
+            g.Server()
+        assert ei.value.code == -6
