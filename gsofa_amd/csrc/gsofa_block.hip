@@ -95,6 +95,7 @@ struct BlkLds {
     uint32_t fbuf[BW][FB];           // staged fills, see flush_fills_w
     unsigned long long s_te[SL];     // per-slot TE of the current mini-group
     unsigned long long s_reach[SL];  // per-slot |R_<(s)|
+    unsigned long long acc[4];       // workgroup totals: fills, TE, |R|, edge checks
     uint32_t src[SL];
     uint32_t qcnt[2];
     uint32_t cursor;
@@ -125,8 +126,11 @@ __device__ __forceinline__ const T& at(const T* base, uint32_t byte_off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
+// Per-lane tallies of the current mini-group (lane j's te / reach / fills
+// belong to slot j; one source's counts fit 32 bits since nnz < 2^31).
 struct TallyB {
-    unsigned long long fills, te, reach, checks;
+    uint32_t fills, te, reach;
+    unsigned long long checks;
 };
 
 // Staged fills are (slot << 25) | global target (n < 2^25); the flush turns
@@ -440,11 +444,12 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
     uint32_t off = p.bstate[2 * blockIdx.x];
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
-    unsigned long long levels = 0, tot_te = 0, tot_reach = 0;
+    unsigned long long levels = 0;
     uint32_t* fb = S.fbuf[w];
     uint32_t fcount = 0;
     for (uint32_t i = tid; i < (uint32_t)(T * SL); i += BB) (&S.cell[0][0])[i] = INF;
     for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
+    if (tid < 4) S.acc[tid] = 0;
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
         S.cursor = S.tcnt = S.held = 0;
@@ -557,28 +562,27 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         if (fcount) flush_fills_w(p, S, fb, fcount);
         fcount = 0;
         // per-row TE and |R_<(s)|: lane j's tallies belong to slot j
-        if (t.te) atomicAdd(&S.s_te[j], t.te);
-        if (t.reach) atomicAdd(&S.s_reach[j], t.reach);
-        t.te = t.reach = 0;
+        if (t.te) atomicAdd(&S.s_te[j], (unsigned long long)t.te);
+        if (t.reach) atomicAdd(&S.s_reach[j], (unsigned long long)t.reach);
+        if (t.fills) atomicAdd(&S.acc[0], (unsigned long long)t.fills);
+        t.te = t.reach = t.fills = 0;
         __syncthreads();
         if (tid < nsrc) {
             p.row_te[S.src[tid]] = S.s_te[tid];
             p.row_reach[S.src[tid]] = S.s_reach[tid];
-            tot_te += S.s_te[tid];
-            tot_reach += S.s_reach[tid];
+            atomicAdd(&S.acc[1], S.s_te[tid]);
+            atomicAdd(&S.acc[2], S.s_reach[tid]);
         }
         off -= W;
         since++;
     }
-    const unsigned long long f = wave_sum64(t.fills), te = wave_sum64(tot_te),
-                             rc = wave_sum64(tot_reach), ch = wave_sum64(t.checks);
-    if (lane == 0) {
-        if (f) aadd64(&p.ctrl->fill_count, f);
-        if (te) aadd64(&p.ctrl->te, te);
-        if (rc) aadd64(&p.ctrl->reach, rc);
-        if (ch) aadd64(&p.ctrl->edge_checks, ch);
-    }
+    if (t.checks) atomicAdd(&S.acc[3], t.checks);
+    __syncthreads();
     if (tid == 0) {
+        if (S.acc[0]) aadd64(&p.ctrl->fill_count, S.acc[0]);
+        if (S.acc[1]) aadd64(&p.ctrl->te, S.acc[1]);
+        if (S.acc[2]) aadd64(&p.ctrl->reach, S.acc[2]);
+        if (S.acc[3]) aadd64(&p.ctrl->edge_checks, S.acc[3]);
         aadd64(&p.ctrl->levels, levels);
         p.bstate[2 * blockIdx.x] = off;
         p.bstate[2 * blockIdx.x + 1] = since;
