@@ -86,7 +86,7 @@ def batch_bench(steps, warmup):
 
     batched:    one plan over all 256 matrices, one persistent launch per step
     one_by_one: the plain path, one plan and one launch per matrix; each plan is
-                created and warmed untimed, then one execute is timed
+                created untimed, then its one execute is timed
     Both run with the matrices resident in HBM; results stay on the device.
     """
     import numpy as np
@@ -107,7 +107,6 @@ def batch_bench(steps, warmup):
     lat = []
     for A in mats:
         with g.Plan(A) as q:
-            q.execute()
             t0 = time.perf_counter()
             q.execute()
             lat.append(time.perf_counter() - t0)
@@ -131,6 +130,57 @@ def batch_bench(steps, warmup):
             "matches_golden": gold_ok}
 
 
+def serve_bench(count, rate, inflight=2, max_batch=64):
+    """C5: a seeded Poisson stream (rate req/s) of C4-family 2D grids and 3D
+    7-point ND grids (side 16..64), submitted at its arrival times to the
+    request server (continuous batching, `inflight` batches on the device) and,
+    as the one-by-one baseline, to a server that runs one request per launch.
+    Latency = submit -> result on the host (includes the upload and the copy
+    of the result)."""
+    import numpy as np
+
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    reqs = cf.stream(count, rate)
+
+    def run(srv):
+        srv.wait(srv.submit(cf.batch_matrix(16, 0)))  # warm-up
+        t0 = time.perf_counter()
+        tickets = []
+        for ta, _, _, A in reqs:
+            now = time.perf_counter() - t0
+            if ta > now:
+                time.sleep(ta - now)
+            tickets.append(srv.submit(A))
+        out = [srv.wait(t) for t in tickets]
+        return out, time.perf_counter() - t0
+
+    with g.Server(max_inflight=inflight, max_batch=max_batch, counts_only=False) as srv:
+        served, span = run(srv)
+    with g.Server(max_inflight=1, max_batch=1) as srv:
+        alone, span1 = run(srv)
+    same = all(a.fill_count == b.fill_count and a.te_alg == b.te_alg and
+               np.array_equal(a.U_col, b.U_col) and np.array_equal(a.L_col, b.L_col)
+               for (a, _), (b, _) in zip(served, alone))
+    lat = np.array([i["latency_ms"] for _, i in served])
+    lat1 = np.array([i["latency_ms"] for _, i in alone])
+    te = sum(r.te_alg for r, _ in served)
+    return {"workload": "Poisson stream, %d requests at %.0f req/s, seed 5: C4-family 2D and "
+                        "3D 7-point ND grids (side 16..64)" % (count, rate),
+            "requests": count, "kinds_3d": sum(k for _, k, _, _ in reqs),
+            "max_inflight": inflight, "max_batch": max_batch,
+            "batches": len({i["batch_id"] for _, i in served}),
+            "served_s": round(span, 3), "served_matrices_per_s": round(count / span, 2),
+            "served_gteps": round(te / span / 1e9, 2),
+            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
+            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
+            "one_by_one_s": round(span1, 3), "one_by_one_matrices_per_s": round(count / span1, 2),
+            "one_by_one_latency_ms_p50": round(float(np.percentile(lat1, 50)), 2),
+            "one_by_one_latency_ms_p99": round(float(np.percentile(lat1, 99)), 2),
+            "speedup_vs_one_by_one": round(span1 / span, 2),
+            "results_equal_alone": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +193,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-matrix batch leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-stream leg")
+    ap.add_argument("--serve-count", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=500.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -216,6 +269,9 @@ def main():
     if not args.no_batch:
         out["batch"] = batch_bench(args.steps, args.warmup)
         log("batch:", json.dumps(out["batch"]))
+    if not args.no_serve:
+        out["serve"] = serve_bench(args.serve_count, args.serve_rate)
+        log("serve:", json.dumps(out["serve"]))
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(A, args.cpu_seconds, threads)
