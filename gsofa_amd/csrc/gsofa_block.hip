@@ -68,8 +68,14 @@ constexpr int BB = GSOFA_BB;         // threads per workgroup
 constexpr int BW = BB / WAVE;        // waves per workgroup
 constexpr int NH = 2 * BW;           // half-waves
 constexpr int SL = kBlockSlots;      // slots per workgroup (lanes of a half-wave)
-constexpr int T = kBlkTable;         // distinct targets per round
-constexpr int HB = 10;               // log2 of the hash slots
+#ifndef GSOFA_T
+#define GSOFA_T kBlkTable
+#endif
+constexpr int T = GSOFA_T;           // distinct targets per round
+#ifndef GSOFA_HB
+#define GSOFA_HB 10
+#endif
+constexpr int HB = GSOFA_HB;         // log2 of the hash slots
 constexpr int H = 1 << HB;           // hash slots (load factor <= T / H)
 constexpr int FB = WAVE;             // staged fills per wave (one ballot's worth)
 constexpr uint32_t INF = 0xFFFFFFFFu;
@@ -81,6 +87,9 @@ constexpr uint32_t HOLE = 0xFFFFFFFFu;   // ent_m of an index that lost its inse
 #endif
 #ifndef GSOFA_KV
 #define GSOFA_KV 4
+#endif
+#ifndef GSOFA_WPE
+#define GSOFA_WPE 4
 #endif
 constexpr int KR = GSOFA_KR;         // table entries relaxed together per half-wave (<= 32)
 constexpr int KV = GSOFA_KV;         // frontier vertices fetched together per half-wave (<= 32)
@@ -427,7 +436,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
     }
 }
 
-__global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(4, 4))) void k_traverse_blk(BlkParams p) {
+__global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, GSOFA_WPE))) void k_traverse_blk(BlkParams p) {
     __shared__ BlkLds S;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();

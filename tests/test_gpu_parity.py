@@ -290,3 +290,15 @@ def test_random_batch_vs_oracle():
         p.execute()
         for b, A in enumerate(mats):
             assert_same_structure(p.fetch_matrix(b), oracle_of(A), max_size=4)
+
+
+def test_autotuner_choice_is_result_neutral(bcsstk18, tmp_path):
+    """Whatever the tuner picks, the
+    full run equals the oracle."""
+    import gsofa_amd as g
+    from gsofa_amd import tune
+    _say("case: autotune bcsstk18")
+    best, timings = tune.tune(bcsstk18, reps=1, cache=str(tmp_path / "t.json"))
+    assert len(timings) == len(tune.BLOCK_CANDIDATES) + len(tune.GRID_CANDIDATES)
+    assert tune.best_options(bcsstk18, cache=str(tmp_path / "t.json")) == best
+    assert_same_structure(g.symbfact(bcsstk18, **best), oracle_of(bcsstk18))

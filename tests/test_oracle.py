@@ -342,3 +342,19 @@ def test_server_argument_checks():
         with pytest.raises(g.GsofaError) as ei:
             g.Server()
         assert ei.value.code == -6
+
+
+def test_tune_signature_and_cache(tmp_path):
+    """Autotuner host logic: signatures bucket sizes, the cache round-trips."""
+    import gsofa_amd as g
+    from gsofa_amd import tune
+    A = g.CsrMatrix.from_rows([[0, 1], [0, 1, 2], [1, 2]])
+    B = g.CsrMatrix.from_rows([[0], [1], [2], [3]])
+    assert tune.signature(A) == tune.signature(A)
+    assert tune.signature([A, B]) != tune.signature(A)
+    path = str(tmp_path / "t.json")
+    assert tune.best_options(A, cache=path) == {}
+    tune._store(path, {tune.signature(A): {"options": {"mode": "block", "grid_blocks": 256}}})
+    assert tune.best_options(A, cache=path) == {"mode": "block", "grid_blocks": 256}
+    for c in tune.BLOCK_CANDIDATES + tune.GRID_CANDIDATES:  # every knob is a symbfact option
+        g._options(**c)
