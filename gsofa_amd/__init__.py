@@ -121,7 +121,7 @@ def write_ascii(A, colbeg_path, colend_path, csr_path):
 
 def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
              max_supernode_size=0, chunk=0, wave_block_threshold=0, device=0, queue_capacity=0,
-             output_capacity=0, counts_only=False, grid_blocks=0, mode=0):
+             output_capacity=0, counts_only=False, grid_blocks=0, mode=0, top_sources=0):
     o = _lib.Options()
     _lib.lib().gsofa_options_default(C.byref(o))
     o.n_virtual_gpu = n_virtual_gpu
@@ -137,6 +137,7 @@ def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
     o.flags = GSOFA_FLAG_COUNTS_ONLY if counts_only else 0
     o.grid_blocks = grid_blocks
     o.mode = {"auto": 0, "block": 1, "grid": 2}.get(mode, mode)
+    o.top_sources = top_sources
     return o
 
 
@@ -155,7 +156,7 @@ class SymbfactResult:
         self.share_ms_min = float(r.share_ms_min)
         self.share_ms_max = float(r.share_ms_max)
         self.chunk = int(r.chunk)
-        self.mode = {1: "block", 2: "grid"}.get(int(r.mode), int(r.mode))
+        self.mode = {1: "block", 2: "grid", 3: "hybrid"}.get(int(r.mode), int(r.mode))
         self.grid_blocks = int(r.grid_blocks)
         self.queue_capacity = int(r.queue_capacity)
         self.levels = int(r.levels)

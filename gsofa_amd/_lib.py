@@ -11,7 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgsofa.so")
 
 GSOFA_FLAG_COUNTS_ONLY = 0x1
-MODE_AUTO, MODE_BLOCK, MODE_GRID = 0, 1, 2
+MODE_AUTO, MODE_BLOCK, MODE_GRID, MODE_HYBRID = 0, 1, 2, 3
 
 # Every entry point include/gsofa.h declares (kept in sync by a CPU test).
 EXPORTS = [
@@ -41,7 +41,8 @@ class Options(C.Structure):
                 ("mem_budget_bytes", C.c_int64), ("max_supernode_size", C.c_int32),
                 ("chunk", C.c_int32), ("wave_block_threshold", C.c_int32), ("device", C.c_int32),
                 ("queue_capacity", C.c_int64), ("output_capacity", C.c_int64),
-                ("flags", C.c_int32), ("grid_blocks", C.c_int32), ("mode", C.c_int32)]
+                ("flags", C.c_int32), ("grid_blocks", C.c_int32), ("mode", C.c_int32),
+                ("top_sources", C.c_int32)]
 
 
 class Result(C.Structure):

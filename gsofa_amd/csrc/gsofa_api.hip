@@ -60,6 +60,15 @@ struct gsofa_plan {
     uint64_t fill_cap = 0;
     Ctrl* d_ctrl = nullptr;
 
+    // hybrid: sources [s_top, n) run first as grid-mode groups [h_g_lo, h_g_hi)
+    // of hC sources (markers, orphans and queues below), the rest in block mode
+    bool hybrid = false;
+    uint32_t s_top = 0;
+    int32_t hC = 0, h_g_lo = 0, h_g_hi = 0;
+    uint32_t hQ = 0, h_nblocks = 0;
+    uint64_t hlab_words = 0;
+    uint32_t* d_hlab = nullptr;
+
     // persistent label/marker state carried across launches
     uint32_t off = 0;
     uint32_t since = 0;
@@ -274,7 +283,7 @@ void build_chunks(gsofa_plan* P) {
     if (P->n_mats == 1) {
         for (int32_t gid = 0; gid < N; gid++) {
             P->share_chunk_off[gid] = (uint32_t)P->h_chunks.size();
-            const uint64_t cnt = share_count(P->n, N, gid);
+            const uint64_t cnt = P->hybrid ? P->s_top : share_count(P->n, N, gid);
             for (uint64_t c = 0; c * kBlockSlots < cnt; c++) {
                 const uint64_t k = cnt - 1 - c * kBlockSlots;
                 BlkChunk ch;
@@ -310,7 +319,7 @@ void build_chunks(gsofa_plan* P) {
 
 int size_block_mode(gsofa_plan* P, int64_t avail) {
     const int32_t N = std::max(1, P->opt.n_virtual_gpu);
-    const uint64_t maxsrc = std::max<uint64_t>(share_count(P->n, N, 0), 1);
+    const uint64_t maxsrc = std::max<uint64_t>(P->hybrid ? P->s_top : share_count(P->n, N, 0), 1);
     const uint64_t nchunks = P->n_mats == 1 ? (maxsrc + kBlockSlots - 1) / kBlockSlots
                                             : std::max<uint64_t>(P->h_chunks.size(), 1);
     uint32_t nb = 0;
@@ -324,16 +333,16 @@ int size_block_mode(gsofa_plan* P, int64_t avail) {
     return GSOFA_OK;
 }
 
-void advance_state(gsofa_plan* P, int32_t gid) {
-    // replicate the kernel's offset evolution for one launch over a share
+// Replicate the grid kernel's offset evolution over groups [g_lo, g_hi) of
+// share gid, C sources per group.
+void advance_groups(gsofa_plan* P, int32_t gid, int32_t C, uint64_t g_lo, uint64_t g_hi) {
     const uint32_t n = (uint32_t)P->n;
     const uint32_t N = (uint32_t)std::max(1, P->opt.n_virtual_gpu);
     const uint32_t loops = 0xFFFFFFFFu / n;
     const uint64_t cnt = share_count(P->n, (int32_t)N, gid);
-    const uint64_t groups = (cnt + P->C - 1) / P->C;
-    for (uint64_t g = 0; g < groups; g++) {
-        const uint64_t k0 = g * (uint64_t)P->C;
-        const uint64_t nsrc = std::min<uint64_t>((uint64_t)P->C, cnt - k0);
+    for (uint64_t g = g_lo; g < g_hi; g++) {
+        const uint64_t k0 = g * (uint64_t)C;
+        const uint64_t nsrc = std::min<uint64_t>((uint64_t)C, cnt - k0);
         const uint64_t size_g = slot_base_h(nsrc, (uint64_t)gid + k0 * N, N);
         if (P->since >= loops) {
             P->off = 0xFFFFFFFFu - n;
@@ -344,6 +353,12 @@ void advance_state(gsofa_plan* P, int32_t gid) {
         P->off -= n;
         P->since++;
     }
+}
+
+void advance_state(gsofa_plan* P, int32_t gid) {
+    const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    const uint64_t cnt = share_count(P->n, N, gid);
+    advance_groups(P, gid, P->C, 0, (cnt + P->C - 1) / P->C);
 }
 
 void release_device(gsofa_plan* P) {
@@ -377,6 +392,7 @@ void release_device(gsofa_plan* P) {
     dfree(P->d_mat_base);
     dfree(P->d_chunks);
     dfree(P->d_row_te);
+    dfree(P->d_hlab);
     dfree(P->d_row_reach);
 }
 
@@ -553,10 +569,25 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
 
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
-    const int64_t avail = (int64_t)((double)freeb * 0.80) - 8 * (int64_t)(n + 1 + P->nnz) -
-                          64ll * (n + 1) - (64ll << 20);
+    int64_t avail = (int64_t)((double)freeb * 0.80) - 8 * (int64_t)(n + 1 + P->nnz) -
+                    64ll * (n + 1) - (64ll << 20);
     P->mode = P->opt.mode == GSOFA_MODE_GRID ? GSOFA_MODE_GRID : GSOFA_MODE_BLOCK;
     P->orphan_words = (uint32_t)(((uint64_t)n + 31) / 32);
+    if (P->mode == GSOFA_MODE_BLOCK && P->opt.top_sources > 0 && n_mats == 1 &&
+        P->opt.n_virtual_gpu == 1 && n > 0) {
+        // hybrid: the top sources as whole grid-mode groups of up to 8192
+        const int64_t K = std::min<int64_t>(P->opt.top_sources, n);
+        P->hybrid = true;
+        P->hC = (int32_t)std::min<int64_t>(K, 8192);
+        P->h_g_hi = (int32_t)(((int64_t)n + P->hC - 1) / P->hC);
+        P->h_g_lo = std::max<int32_t>(0, P->h_g_hi - (int32_t)((K + P->hC - 1) / P->hC));
+        P->s_top = (uint32_t)P->h_g_lo * (uint32_t)P->hC;
+        P->hQ = P->opt.queue_capacity > 0 ? (uint32_t)std::min<int64_t>(P->opt.queue_capacity, 1ll << 28)
+                                          : default_queue(n, P->hC);
+        P->hlab_words = label_words(n, 1, P->hC, 0, 0);
+        avail -= 4 * (int64_t)P->hlab_words + 4 * (int64_t)P->hC * n + 4ll * P->hC * P->orphan_words +
+                 16 * (int64_t)P->hQ;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, P->device) != hipSuccess) {
         (void)hipGetLastError();
@@ -583,6 +614,13 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         e |= dalloc(&P->d_row_te, (size_t)n);
         e |= dalloc(&P->d_row_reach, (size_t)n);
         P->lab_words = slots * (size_t)P->W;
+        if (P->hybrid) {
+            e |= dalloc(&P->d_hlab, (size_t)P->hlab_words);
+            e |= dalloc(&P->d_mark, (size_t)P->hC * (size_t)n);
+            e |= dalloc(&P->d_orphan, (size_t)P->hC * P->orphan_words);
+            e |= dalloc(&P->d_q0, (size_t)P->hQ);
+            e |= dalloc(&P->d_q1, (size_t)P->hQ);
+        }
     } else {
         Sizing sz;
         rc = choose_sizing(A, &P->opt, P->nnz, std::max<int64_t>(avail, 1 << 20), &sz);
@@ -623,11 +661,13 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     if (P->nnz)
         ok = ok && hipMemcpyAsync(P->d_ci, ci.data(), sizeof(int32_t) * P->nnz, hipMemcpyHostToDevice,
                                   P->stream) == hipSuccess;
-    if (P->mode == GSOFA_MODE_GRID) {
-        ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream) == hipSuccess;
-        ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * (size_t)P->C * P->orphan_words,
+    if (P->mode == GSOFA_MODE_GRID || P->hybrid) {
+        const size_t Cm = P->hybrid ? (size_t)P->hC : (size_t)P->C;
+        ok = ok && hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * Cm * n, P->stream) == hipSuccess;
+        ok = ok && hipMemsetAsync(P->d_orphan, 0, sizeof(uint32_t) * Cm * P->orphan_words,
                                   P->stream) == hipSuccess;
-    } else {
+    }
+    if (P->mode == GSOFA_MODE_BLOCK) {
         ok = ok && hipMemsetAsync(P->d_mask, 0, sizeof(uint32_t) * (size_t)P->nblocks * 2 * P->W,
                                   P->stream) == hipSuccess;
     }
@@ -660,7 +700,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     P->hw = 0;
     P->epoch = 0;
 
-    if (P->mode == GSOFA_MODE_GRID) {
+    if (P->mode == GSOFA_MODE_GRID || P->hybrid) {
         // persistent grid: as many blocks as can be co-resident
         int occ = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, traverse_kernel_ptr(), traverse_block_size(),
@@ -673,8 +713,11 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
             return GSOFA_ERR_HIP;
         }
         uint32_t maxblocks = (uint32_t)occ * (uint32_t)prop.multiProcessorCount;
-        P->nblocks = P->opt.grid_blocks > 0 ? std::min<uint32_t>((uint32_t)P->opt.grid_blocks, maxblocks)
-                                             : std::min<uint32_t>(maxblocks, (uint32_t)prop.multiProcessorCount);
+        const uint32_t gb = P->opt.grid_blocks > 0 && !P->hybrid
+                                ? std::min<uint32_t>((uint32_t)P->opt.grid_blocks, maxblocks)
+                                : std::min<uint32_t>(maxblocks, (uint32_t)prop.multiProcessorCount);
+        if (P->hybrid) P->h_nblocks = gb;
+        else P->nblocks = gb;
     }
     int rate_khz = 0;
     if (hipDeviceGetAttribute(&rate_khz, hipDeviceAttributeWallClockRate, P->device) != hipSuccess ||
@@ -728,6 +771,45 @@ int gsofa_batch_create(const gsofa_matrix* mats, int32_t count, const gsofa_opti
 
 int32_t gsofa_plan_matrix_count(const gsofa_plan* P) { return P ? P->n_mats : 0; }
 
+}  // extern "C"
+
+// Grid-mode launch parameters for every group of share gid.
+static TravParams grid_params(const gsofa_plan* P, int32_t gid) {
+    const int32_t n = P->n;
+    const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    const uint64_t cnt = share_count(n, N, gid);
+    TravParams tp{};
+    tp.rp = P->d_rp;
+    tp.ci = P->d_ci;
+    tp.n = n;
+    tp.n_gpu = N;
+    tp.gpu_id = gid;
+    tp.chunk = P->C;
+    tp.n_groups = P->C > 0 ? (int32_t)((cnt + P->C - 1) / P->C) : 0;
+    tp.first_group = 0;
+    tp.lab = P->d_lab;
+    tp.lab_words = P->lab_words;
+    tp.mark = P->d_mark;
+    tp.qbuf0 = P->d_q0;
+    tp.qbuf1 = P->d_q1;
+    tp.qcap = P->Q;
+    tp.orphan_words = P->orphan_words;
+    tp.orphan = P->d_orphan;
+    tp.fills = P->d_fills;
+    tp.fill_cap = P->fill_cap;
+    tp.ctrl = P->d_ctrl;
+    tp.nblocks = P->nblocks;
+    tp.block_threshold = P->opt.wave_block_threshold;
+    tp.off_start = P->off;
+    tp.groups_since_init = P->since;
+    tp.lab_hw = P->hw;
+    tp.stamp_base = P->epoch * (uint32_t)n;
+    tp.spin_ticks = P->spin_ticks;
+    return tp;
+}
+
+extern "C" {
+
 int gsofa_plan_execute(gsofa_plan* P) {
     if (!P) return GSOFA_ERR_INVALID;
     HIPCHK(hipSetDevice(P->device));
@@ -760,12 +842,12 @@ int gsofa_plan_execute(gsofa_plan* P) {
         HIPCHK(hipEventRecord(P->ev[0], P->stream));
         for (size_t si = 0; si < shares.size(); si++) {
             const int32_t gid = shares[si];
-            const uint64_t cnt = share_count(n, N, gid);
             // grid mode: stamps must increase per marker slot: move to a fresh
             // stamp range (or clear the markers when the 32-bit range is used up)
-            if (P->mode == GSOFA_MODE_GRID && !P->marks_clean) {
+            if ((P->mode == GSOFA_MODE_GRID || P->hybrid) && !P->marks_clean) {
                 if ((uint64_t)(P->epoch + 2) * (uint64_t)n >= 0xFFFFFFFFull) {
-                    HIPCHK(hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * (size_t)P->C * n, P->stream));
+                    const size_t Cm = P->hybrid ? (size_t)P->hC : (size_t)P->C;
+                    HIPCHK(hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * Cm * n, P->stream));
                     P->epoch = 0;
                 } else {
                     P->epoch++;
@@ -793,37 +875,23 @@ int gsofa_plan_execute(gsofa_plan* P) {
                 bp.bstate = P->d_bstate;
                 HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
                 HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+                if (P->hybrid && P->h_g_hi > P->h_g_lo) {  // the top groups, whole device each
+                    TravParams tp = grid_params(P, gid);
+                    tp.chunk = P->hC;
+                    tp.first_group = P->h_g_lo;
+                    tp.n_groups = P->h_g_hi - P->h_g_lo;
+                    tp.lab = P->d_hlab;
+                    tp.lab_words = P->hlab_words;
+                    tp.qcap = P->hQ;
+                    tp.nblocks = P->h_nblocks;
+                    HIPCHK(launch_traverse(tp, P->stream, P->coop));
+                    advance_groups(P, gid, P->hC, (uint64_t)P->h_g_lo, (uint64_t)P->h_g_hi);
+                }
                 if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->blk_lds, P->stream));
                 HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
                 continue;
             }
-            TravParams tp{};
-            tp.rp = P->d_rp;
-            tp.ci = P->d_ci;
-            tp.n = n;
-            tp.n_gpu = N;
-            tp.gpu_id = gid;
-            tp.chunk = P->C;
-            tp.n_groups = (int32_t)((cnt + P->C - 1) / P->C);
-            tp.first_group = 0;
-            tp.lab = P->d_lab;
-            tp.lab_words = P->lab_words;
-            tp.mark = P->d_mark;
-            tp.qbuf0 = P->d_q0;
-            tp.qbuf1 = P->d_q1;
-            tp.qcap = P->Q;
-            tp.orphan_words = P->orphan_words;
-            tp.orphan = P->d_orphan;
-            tp.fills = P->d_fills;
-            tp.fill_cap = P->fill_cap;
-            tp.ctrl = P->d_ctrl;
-            tp.nblocks = P->nblocks;
-            tp.block_threshold = P->opt.wave_block_threshold;
-            tp.off_start = P->off;
-            tp.groups_since_init = P->since;
-            tp.lab_hw = P->hw;
-            tp.stamp_base = P->epoch * (uint32_t)n;
-            tp.spin_ticks = P->spin_ticks;
+            TravParams tp = grid_params(P, gid);
             HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
             if (tp.n_groups > 0) HIPCHK(launch_traverse(tp, P->stream, P->coop));
             HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
@@ -928,7 +996,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->share_ms_min = P->share_min;
     r->share_ms_max = P->share_max;
     r->chunk = P->C;
-    r->mode = P->mode;
+    r->mode = P->hybrid ? GSOFA_MODE_HYBRID : P->mode;
     r->grid_blocks = (int32_t)P->nblocks;
     r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->W : (int64_t)P->Q;
     r->n_super = -1;
@@ -952,7 +1020,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         return GSOFA_ERR_ALLOC;
     }
     HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
-    if (P->mode == GSOFA_MODE_BLOCK) {
+    if (P->mode == GSOFA_MODE_BLOCK && !P->hybrid) {
         r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
         r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
         if (!r->row_te || !r->row_reach) {

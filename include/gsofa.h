@@ -83,6 +83,9 @@ typedef struct gsofa_options {
     int32_t flags;                /* GSOFA_FLAG_*                                            */
     int32_t grid_blocks;          /* persistent grid size; 0 = occupancy-bounded maximum    */
     int32_t mode;                 /* GSOFA_MODE_*                                           */
+    int32_t top_sources;          /* block mode, one matrix, one share: the top sources     */
+                                  /* (rounded up to whole groups) first run as grid-mode    */
+                                  /* groups (GSOFA_MODE_HYBRID); 0 = none                   */
 } gsofa_options;
 
 /* Execution modes of the traversal (identical results):
@@ -92,10 +95,14 @@ typedef struct gsofa_options {
  * GRID:  one group of chunk sources at a time; all workgroups expand each
  *        frontier level together, separated by a grid-wide generation barrier
  *        (the reference's persistent kernel, without_space_optimization/
- *        symbfact.cuh:706-962). */
+ *        symbfact.cuh:706-962).
+ * HYBRID (reported, chosen with top_sources): the deepest sources, whose
+ *        traversals are long chains of wide levels, run as GRID groups that
+ *        spread every level over the whole device; the rest run in BLOCK mode. */
 #define GSOFA_MODE_AUTO 0
 #define GSOFA_MODE_BLOCK 1
 #define GSOFA_MODE_GRID 2
+#define GSOFA_MODE_HYBRID 3
 
 void gsofa_options_default(gsofa_options* opt);
 
@@ -124,13 +131,13 @@ typedef struct gsofa_result {
     double share_ms_min;  /* fastest / slowest share (reference log, symbfact.cuh:2530) */
     double share_ms_max;
     int32_t chunk;        /* concurrent sources actually used                          */
-    int32_t mode;         /* GSOFA_MODE_BLOCK or GSOFA_MODE_GRID                       */
+    int32_t mode;         /* GSOFA_MODE_BLOCK, GSOFA_MODE_GRID or GSOFA_MODE_HYBRID    */
     int32_t grid_blocks;  /* persistent grid actually used                             */
     int64_t queue_capacity;
     uint64_t levels;      /* frontier levels summed over groups                        */
     uint64_t rescans;     /* queue-overflow drain-and-resume passes                    */
-    int64_t* row_te;      /* n: TE of each processed row (block mode; NULL otherwise)  */
-    int64_t* row_reach;   /* n: |R_<(s)| of each processed row (block mode)            */
+    int64_t* row_te;      /* n: TE of each processed row (block mode only; else NULL)  */
+    int64_t* row_reach;   /* n: |R_<(s)| of each processed row (block mode only)       */
 } gsofa_result;
 
 void gsofa_result_free(gsofa_result* r);

@@ -302,3 +302,36 @@ def test_autotuner_choice_is_result_neutral(bcsstk18, tmp_path):
     assert len(timings) == len(tune.BLOCK_CANDIDATES) + len(tune.GRID_CANDIDATES)
     assert tune.best_options(bcsstk18, cache=str(tmp_path / "t.json")) == best
     assert_same_structure(g.symbfact(bcsstk18, **best), oracle_of(bcsstk18))
+
+
+@pytest.mark.parametrize("top", [1, 100, 4096, 9000, 20000])
+def test_hybrid_top_sources(bcsstk18, top):
+    """The top sources as grid-mode groups, the rest in block mode: bit-exact,
+    repeated executes included (both label states carry over)."""
+    import gsofa_amd as g
+    o = oracle_of(bcsstk18)
+    _say("case: bcsstk18 hybrid top", top)
+    with g.Plan(bcsstk18, top_sources=top) as p:
+        for it in range(2):
+            p.execute()
+            r = p.fetch()
+            assert r.mode == "hybrid" and r.row_te is None
+            assert_same_structure(r, o)
+
+
+def test_hybrid_random_and_c2():
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    rng = np.random.default_rng(91)
+    for trial in range(6):
+        n = int(rng.integers(2, 300))
+        A, _ = random_matrix(rng, n, float(rng.choice([0.01, 0.05])))
+        top = int(rng.integers(1, n + 1))
+        _say("case: random hybrid n", n, "top", top)
+        assert_same_structure(g.symbfact(A, top_sources=top), oracle_of(A))
+    gold = _golden("c2")
+    _say("case: C2 48^3 ND hybrid top 8192")
+    r = g.symbfact(cf.config2(), top_sources=8192)
+    rec = record_of(r)
+    for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
+        assert rec[k] == gold[k], k
