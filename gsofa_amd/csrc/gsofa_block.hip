@@ -77,7 +77,6 @@ constexpr int T = GSOFA_T;           // distinct targets per round
 #endif
 constexpr int HB = GSOFA_HB;         // log2 of the hash slots
 constexpr int H = 1 << HB;           // hash slots (load factor <= T / H)
-constexpr int FB = WAVE;             // staged fills per wave (one ballot's worth)
 constexpr uint32_t INF = 0xFFFFFFFFu;
 constexpr uint32_t NONE = 0xFFFFFFFFu;   // no table entry (edge not in this group)
 constexpr uint32_t FULL = 0xFFFFFFFEu;   // table full: retry the edge next round
@@ -101,7 +100,6 @@ struct BlkLds {
     unsigned long long hkey[H];      // ((m + 1) << 32) | table index; 0 = empty
     uint32_t ent_m[T];               // table index -> target (window-local) or HOLE
     uint4 eidx[NH][8];               // table index of each lane's edge (32 per half-wave)
-    uint32_t fbuf[BW][FB];           // staged fills, see flush_fills_w
     unsigned long long s_te[SL];     // per-slot TE of the current mini-group
     unsigned long long s_reach[SL];  // per-slot |R_<(s)|
     unsigned long long acc[4];       // workgroup totals: fills, TE, |R|, edge checks
@@ -141,38 +139,6 @@ struct TallyB {
     uint32_t fills, te, reach;
     unsigned long long checks;
 };
-
-// Staged fills are (slot << 25) | global target (n < 2^25); the flush turns
-// them into (source << 32) | target with the slot's source.
-__device__ void flush_fills_w(const BlkParams& p, const BlkLds& S, const uint32_t* fb, uint32_t cnt) {
-    wave_sync();
-    const uint32_t lane = __lane_id();
-    uint64_t pos = 0;
-    if (lane == 0) pos = aadd64(&p.ctrl->fill_pos, cnt);
-    pos = bcast0_64(pos);
-    for (uint32_t t = lane; t < cnt; t += WAVE) {
-        const uint64_t at = pos + t;
-        const uint32_t f = fb[t];
-        if (at < p.fill_cap) p.fills[at] = ((uint64_t)S.src[f >> 25] << 32) | (f & 0x1FFFFFFu);
-    }
-    wave_sync();
-}
-
-// Wave-uniform call: every lane of the wave must be active.  One call stages
-// at most 64 fills, so a 64-entry buffer never overflows.
-__device__ __forceinline__ void stage_fill(const BlkParams& p, const BlkLds& S, uint32_t* fb,
-                                           uint32_t& fcount, bool fill, uint32_t entry) {
-    const uint64_t fm = __ballot(fill);
-    if (fm) {
-        const uint32_t c = (uint32_t)__popcll(fm);
-        if (fcount + c > (uint32_t)FB) {
-            flush_fills_w(p, S, fb, fcount);
-            fcount = 0;
-        }
-        if (fill) fb[fcount + rank_in(fm)] = entry;
-        fcount += c;
-    }
-}
 
 // Table index of target m in this round, inserting it if new; FULL when the
 // table has no room left.  Linear probing over H slots; an index allocated for
@@ -352,8 +318,8 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
 // entries: lane k loads entry k's CSR range and next-level mask, then the
 // half issues all 32 label-line loads before using any, so a batch costs one
 // memory round trip.  Wave-uniform control flow.
-__device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_src,
-                            uint32_t* fb, uint32_t& fcount, TallyB& t) {
+__device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_src, uint32_t my_srcg,
+                            TallyB& t) {
     const BlkParams& p = *B.p;
     const uint32_t lane = __lane_id();
     const uint32_t half = lane >> 5;
@@ -388,7 +354,8 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
             lm[u] = 0u;
             if (mu != HOLE) lm[u] = at(B.lab0, (mu * SL + j) << 2);
         }
-        uint32_t mybits = 0;  // lane k: slots that push entry k's target
+        uint32_t mybits = 0;    // lane k: slots that push entry k's target
+        uint32_t fillbits = 0;  // lane j: bit u = slot j gains entry u's target
 #pragma unroll
         for (int u = 0; u < KR; u++) {
             const uint32_t mu = from_lane(mk, u, half);
@@ -418,8 +385,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
             }
             const uint32_t pb = (uint32_t)(__ballot(push) >> (32u * half));
             if (j == (uint32_t)u) mybits = pb;
-            if (fill) t.fills++;
-            stage_fill(p, S, fb, fcount, fill, (j << 25) | (mu + B.wb));
+            if (fill) fillbits |= 1u << u;
         }
         // next-level slot masks and queue entries, one lane per entry (each
         // target is unique in the round, so no other lane touches its mask)
@@ -432,6 +398,20 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
             if (lane == 0) qb = atomicAdd(&S.qcnt[nbuf], (uint32_t)__popcll(em));
             qb = bcast0(qb);
             if (enq) qn[qb + rank_in(em)] = make_uint4(mk, (uint32_t)d0, (uint32_t)d1, 0u);
+        }
+        // fills of the batch, (source << 32) | target: one reservation per wave
+        const uint32_t nf = (uint32_t)__popc(fillbits);
+        if (__ballot(nf != 0u)) {
+            t.fills += nf;
+            const uint32_t incl = wave_incl_scan(nf);
+            const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+            uint64_t pos = 0;
+            if (lane == 0) pos = aadd64(&p.ctrl->fill_pos, total);
+            pos = bcast0_64(pos) + (incl - nf);
+            for (uint32_t b = fillbits; b; b &= b - 1u, pos++) {
+                const uint32_t mu = S.ent_m[eb + (uint32_t)__builtin_ctz(b)];  // per-lane index
+                if (pos < p.fill_cap) p.fills[pos] = ((uint64_t)my_srcg << 32) | (mu + B.wb);
+            }
         }
     }
 }
@@ -454,8 +434,6 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
     unsigned long long levels = 0;
-    uint32_t* fb = S.fbuf[w];
-    uint32_t fcount = 0;
     for (uint32_t i = tid; i < (uint32_t)(T * SL); i += BB) (&S.cell[0][0])[i] = INF;
     for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
     if (tid < 4) S.acc[tid] = 0;
@@ -486,7 +464,8 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
         B.stale = off + W;
         // this lane's source (window-local); lanes of slots >= nsrc never
         // receive a mask bit
-        const uint32_t my_src = S.src[j] - B.wb;
+        const uint32_t my_srcg = S.src[j];
+        const uint32_t my_src = my_srcg - B.wb;
 
         // ---- seeding (symbfact.cuh:639-721) by wave 0 alone, slot after slot,
         // so the mask read-modify-writes of different slots cannot race.
@@ -549,7 +528,7 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
                 const bool more = S.held != 0u || S.cursor < size;
-                relax_phase(B, S, nbuf, my_src, fb, fcount, t);
+                relax_phase(B, S, nbuf, my_src, my_srcg, t);
                 __syncthreads();
                 for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
                 if (tid == 0) {
@@ -567,9 +546,6 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
             cur = nbuf;
             levels++;
         }
-        // staged fills name their slot: flush before the slots change
-        if (fcount) flush_fills_w(p, S, fb, fcount);
-        fcount = 0;
         // per-row TE and |R_<(s)|: lane j's tallies belong to slot j
         if (t.te) atomicAdd(&S.s_te[j], (unsigned long long)t.te);
         if (t.reach) atomicAdd(&S.s_reach[j], (unsigned long long)t.reach);
