@@ -176,12 +176,25 @@ __device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
     return res;
 }
 
-// Lane u's value of x within the caller's half-wave (u uniform).  Reads both
-// candidate lanes regardless of EXEC, so x must be written in every lane.
-__device__ __forceinline__ uint32_t from_lane(uint32_t x, uint32_t u, uint32_t half) {
-    const uint32_t a = __builtin_amdgcn_readlane(x, u);
-    const uint32_t b = __builtin_amdgcn_readlane(x, u + 32u);
-    return half ? b : a;
+// Lane u's value of x within the caller's half-wave: a swizzle broadcast
+// inside each group of 32 lanes (and-mask 0, or-mask u).  u must fold to a
+// constant (the swizzle pattern is an immediate); the source lane must be
+// active.
+template <int U>
+__device__ __forceinline__ uint32_t bcast32(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, U << 5);
+}
+__device__ __forceinline__ uint32_t from_lane(uint32_t x, uint32_t u, uint32_t /*half*/) {
+    switch (u) {
+#define GSOFA_BC(k) \
+    case k: return bcast32<k>(x);
+        GSOFA_BC(0) GSOFA_BC(1) GSOFA_BC(2) GSOFA_BC(3) GSOFA_BC(4) GSOFA_BC(5) GSOFA_BC(6) GSOFA_BC(7)
+        GSOFA_BC(8) GSOFA_BC(9) GSOFA_BC(10) GSOFA_BC(11) GSOFA_BC(12) GSOFA_BC(13) GSOFA_BC(14) GSOFA_BC(15)
+        GSOFA_BC(16) GSOFA_BC(17) GSOFA_BC(18) GSOFA_BC(19) GSOFA_BC(20) GSOFA_BC(21) GSOFA_BC(22) GSOFA_BC(23)
+        GSOFA_BC(24) GSOFA_BC(25) GSOFA_BC(26) GSOFA_BC(27) GSOFA_BC(28) GSOFA_BC(29) GSOFA_BC(30) GSOFA_BC(31)
+#undef GSOFA_BC
+        default: return 0u;
+    }
 }
 
 // Frontier vertices held by a half-wave across rounds.  Lane k of the half
@@ -364,6 +377,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
                 c = S.cell[eb + u][j];
                 S.cell[eb + u][j] = INF;  // ready for the next round
             }
+            const uint32_t du = from_lane(dd, u, half);  // broadcast while every lane is active
             bool push = false, fill = false;
             if (c != INF) {
                 const uint32_t o = (mu * SL + j) << 2;
@@ -377,7 +391,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
                     push = mu < my_src;
                     if (lm[u] >= B.stale) {  // first reach: |R_<(s)| and TE_alg
                         t.reach++;
-                        t.te += from_lane(dd, u, half);
+                        t.te += du;
                     }
                     const uint32_t mo = mu + off;
                     fill = lm[u] >= mo && c < mo;  // label crosses below m: new L entry
