@@ -371,30 +371,26 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
         uint32_t fillbits = 0;  // lane j: bit u = slot j gains entry u's target
 #pragma unroll
         for (int u = 0; u < KR; u++) {
+            // broadcasts while every lane is active
             const uint32_t mu = from_lane(mk, u, half);
-            uint32_t c = INF;
-            if (mu != HOLE) {
-                c = S.cell[eb + u][j];
-                S.cell[eb + u][j] = INF;  // ready for the next round
-            }
-            const uint32_t du = from_lane(dd, u, half);  // broadcast while every lane is active
+            const uint32_t du = from_lane(dd, u, half);
+            // hole and unused cells are INF: no entry check needed
+            const uint32_t c = S.cell[eb + u][j];
+            S.cell[eb + u][j] = INF;  // ready for the next round
             bool push = false, fill = false;
             if (c != INF) {
-                const uint32_t o = (mu * SL + j) << 2;
-                if (mu > my_src) {  // m above the source: U entry on first touch (symbfact.cuh:992-1009)
-                    if (lm[u] >= B.stale) {
-                        at(B.lab0, o) = c;
-                        fill = true;
-                    }
-                } else if (lm[u] > c) {  // relaxation (symbfact.cuh:1018-1063)
-                    at(B.lab0, o) = c;
-                    push = mu < my_src;
-                    if (lm[u] >= B.stale) {  // first reach: |R_<(s)| and TE_alg
-                        t.reach++;
-                        t.te += du;
-                    }
+                const uint32_t l = lm[u];
+                const bool up = mu > my_src;  // above the source (symbfact.cuh:992-1009)
+                const bool first = l >= B.stale;
+                // U entry: first touch; below: the relaxation of symbfact.cuh:1018-1063
+                if (up ? first : (l > c)) {
+                    at(B.lab0, (mu * SL + j) << 2) = c;
                     const uint32_t mo = mu + off;
-                    fill = lm[u] >= mo && c < mo;  // label crosses below m: new L entry
+                    fill = up || (l >= mo && c < mo);  // below: the label crosses below m
+                    push = !up;  // the source itself never decreases (its label is off)
+                    const bool fr = !up && first;  // first reach: |R_<(s)| and TE_alg
+                    t.reach += fr ? 1u : 0u;
+                    t.te += fr ? du : 0u;
                 }
             }
             const uint32_t pb = (uint32_t)(__ballot(push) >> (32u * half));
