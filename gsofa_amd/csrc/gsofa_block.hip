@@ -146,6 +146,10 @@ struct TallyB {
 __device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
     const unsigned long long key = (unsigned long long)(m + 1u) << 32;
     uint32_t s = (m * 2654435761u) >> (32 - HB);
+    {
+        const unsigned long long cur = S.hkey[s];  // usual case: present at its first slot
+        if ((cur & 0xFFFFFFFF00000000ull) == key) return (uint32_t)cur;
+    }
     uint32_t mine = NONE, res = FULL;
     for (int probe = 0; probe < H; probe++) {
         unsigned long long cur = S.hkey[s];
@@ -293,15 +297,32 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
                 const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
                 wave_sync();
                 if (hv.cost[u] != INF && okm) {
-                    // four indices per LDS read; FULL edges (okm clear) are skipped
-                    const uint32_t top = 32u - (uint32_t)__builtin_clz(okm);
-                    for (uint32_t k4 = 0; k4 < top; k4 += 4) {
-                        const uint4 id = S.eidx[h][k4 >> 2];
-                        const uint32_t bits = okm >> k4;
-                        if (bits & 1u) atomicMin(&S.cell[id.x][j], hv.cost[u]);
-                        if (bits & 2u) atomicMin(&S.cell[id.y][j], hv.cost[u]);
-                        if (bits & 4u) atomicMin(&S.cell[id.z][j], hv.cost[u]);
-                        if (bits & 8u) atomicMin(&S.cell[id.w][j], hv.cost[u]);
+                    const uint32_t cu = hv.cost[u];
+                    if ((okm & (okm + 1u)) == 0u) {
+                        // usual case, edges 0 .. ne-1 all placed: four indices
+                        // per LDS read, no per-edge tests
+                        const uint32_t ne = (uint32_t)__popc(okm);
+                        uint32_t k4 = 0;
+                        for (; k4 + 4u <= ne; k4 += 4u) {
+                            const uint4 id = S.eidx[h][k4 >> 2];
+                            atomicMin(&S.cell[id.x][j], cu);
+                            atomicMin(&S.cell[id.y][j], cu);
+                            atomicMin(&S.cell[id.z][j], cu);
+                            atomicMin(&S.cell[id.w][j], cu);
+                        }
+                        const uint32_t* ids = reinterpret_cast<const uint32_t*>(S.eidx[h]);
+                        for (; k4 < ne; k4++) atomicMin(&S.cell[ids[k4]][j], cu);
+                    } else {
+                        // the table filled up part-way: skip the edges not placed
+                        const uint32_t top = 32u - (uint32_t)__builtin_clz(okm);
+                        for (uint32_t k4 = 0; k4 < top; k4 += 4) {
+                            const uint4 id = S.eidx[h][k4 >> 2];
+                            const uint32_t bits = okm >> k4;
+                            if (bits & 1u) atomicMin(&S.cell[id.x][j], cu);
+                            if (bits & 2u) atomicMin(&S.cell[id.y][j], cu);
+                            if (bits & 4u) atomicMin(&S.cell[id.z][j], cu);
+                            if (bits & 8u) atomicMin(&S.cell[id.w][j], cu);
+                        }
                     }
                 }
                 wave_sync();
