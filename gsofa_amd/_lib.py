@@ -42,7 +42,8 @@ class Options(C.Structure):
                 ("chunk", C.c_int32), ("wave_block_threshold", C.c_int32), ("device", C.c_int32),
                 ("queue_capacity", C.c_int64), ("output_capacity", C.c_int64),
                 ("flags", C.c_int32), ("grid_blocks", C.c_int32), ("mode", C.c_int32),
-                ("top_sources", C.c_int32)]
+                ("top_sources", C.c_int32),
+                ("block_slots", C.c_int32)]
 
 
 class Result(C.Structure):

@@ -273,22 +273,23 @@ int choose_sizing(const gsofa_matrix* A, const gsofa_options* opt, int64_t nnz_p
 }
 
 
-// Work table: chunks of up to kBlockSlots sources of one matrix, in the order
+// Work table: chunks of up to S (slots per workgroup) sources of one matrix, in the order
 // they are handed out (decreasing global source).  Shares of a single matrix
 // get consecutive ranges of the table.
 void build_chunks(gsofa_plan* P) {
     const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    const uint32_t S = P->S;
     P->h_chunks.clear();
     P->share_chunk_off.assign((size_t)N + 1, 0);
     if (P->n_mats == 1) {
         for (int32_t gid = 0; gid < N; gid++) {
             P->share_chunk_off[gid] = (uint32_t)P->h_chunks.size();
             const uint64_t cnt = P->hybrid ? P->s_top : share_count(P->n, N, gid);
-            for (uint64_t c = 0; c * kBlockSlots < cnt; c++) {
-                const uint64_t k = cnt - 1 - c * kBlockSlots;
+            for (uint64_t c = 0; c * S < cnt; c++) {
+                const uint64_t k = cnt - 1 - c * S;
                 BlkChunk ch;
                 ch.first = (uint32_t)(gid + k * (uint64_t)N);
-                ch.count = (uint32_t)std::min<uint64_t>(kBlockSlots, cnt - c * kBlockSlots);
+                ch.count = (uint32_t)std::min<uint64_t>(S, cnt - c * S);
                 ch.stride = (uint32_t)N;
                 ch.mat = 0;
                 P->h_chunks.push_back(ch);
@@ -299,10 +300,10 @@ void build_chunks(gsofa_plan* P) {
     }
     for (int32_t b = P->n_mats - 1; b >= 0; b--) {
         const uint32_t base = P->h_mat_base[b], nb = P->h_mat_base[b + 1] - base;
-        for (uint32_t c = 0; c * kBlockSlots < nb; c++) {
+        for (uint32_t c = 0; c * S < nb; c++) {
             BlkChunk ch;
-            ch.first = base + nb - 1 - c * kBlockSlots;
-            ch.count = std::min<uint32_t>(kBlockSlots, nb - c * kBlockSlots);
+            ch.first = base + nb - 1 - c * S;
+            ch.count = std::min<uint32_t>(S, nb - c * S);
             ch.stride = 1;
             ch.mat = (uint32_t)b;
             P->h_chunks.push_back(ch);
@@ -320,14 +321,14 @@ void build_chunks(gsofa_plan* P) {
 int size_block_mode(gsofa_plan* P, int64_t avail) {
     const int32_t N = std::max(1, P->opt.n_virtual_gpu);
     const uint64_t maxsrc = std::max<uint64_t>(P->hybrid ? P->s_top : share_count(P->n, N, 0), 1);
-    const uint64_t nchunks = P->n_mats == 1 ? (maxsrc + kBlockSlots - 1) / kBlockSlots
+    const uint32_t S = P->S;
+    const uint64_t nchunks = P->n_mats == 1 ? (maxsrc + S - 1) / S
                                             : std::max<uint64_t>(P->h_chunks.size(), 1);
     uint32_t nb = 0;
-    const int rc = gsofa::block_grid(P->opt, nchunks, P->W, avail, &nb);
+    const int rc = gsofa::block_grid(P->opt, (int)S, nchunks, P->W, avail, &nb);
     if (rc) return rc;
     P->nblocks = nb;
-    P->S = kBlockSlots;
-    P->C = (int32_t)(nb * kBlockSlots);
+    P->C = (int32_t)(nb * S);
     P->Q = 0;
     P->blk_lds = 0;  // all LDS of the block kernel is static
     return GSOFA_OK;
@@ -427,10 +428,10 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 
 namespace gsofa {
 
-// Block mode: per workgroup, labels [W][32] + two masks + two 16-byte-entry
-// queues = 168 W bytes (+ 8 of state).
-int64_t block_bytes(uint32_t nb, uint32_t W) {
-    return (int64_t)nb * ((4ll * kBlockSlots + 8 + 32) * W + 8);
+// Block mode: per workgroup, labels [W][S] + two masks + two 16-byte-entry
+// queues = (4 S + 40) W bytes (+ 8 of state): 168 W at 32 slots.
+int64_t block_bytes(uint32_t nb, uint32_t W, int slots) {
+    return (int64_t)nb * ((4ll * slots + 8 + 32) * W + 8);
 }
 
 // Everything else a block-mode plan allocates: CSR, fill stream, per-row
@@ -443,13 +444,50 @@ int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchun
 
 uint64_t default_fill_cap(int64_t nnz) { return std::max<uint64_t>(64 * (uint64_t)nnz, 1 << 20); }
 
+// Slots per workgroup when the options leave it open (block_slots = 0).
+// Batches take kBatchSlots: within a few percent of the best on batches of
+// small 2D meshes (where 8 is best) and the best on 3D meshes.  A single
+// matrix with nsrc sources takes the largest of 32, 16, 8 whose chunks still
+// give every co-resident workgroup one.
+int choose_slots(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* slots) {
+    if (o.block_slots != 0) {
+        if (!blk_slots_valid(o.block_slots)) return GSOFA_ERR_INVALID;
+        *slots = o.block_slots;
+        return GSOFA_OK;
+    }
+    if (n_mats > 1) {
+        *slots = kBatchSlots;
+        return GSOFA_OK;
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
+    for (int S = kBlockSlots; S > 8; S /= 2) {
+        int occ = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(S), blk_block_size(), 0) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            return GSOFA_ERR_HIP;
+        }
+        if ((nsrc + S - 1) / S >= (uint64_t)std::max(occ, 1) * (uint64_t)cus) {
+            *slots = S;
+            return GSOFA_OK;
+        }
+    }
+    *slots = 8;
+    return GSOFA_OK;
+}
+
 // As many workgroups as can be co-resident (or as the options ask: chunk =
 // concurrent sources, grid_blocks), at most one per chunk, shrunk until the
 // per-workgroup arrays fit `avail` bytes (and opt.mem_budget_bytes if set).
-int block_grid(const gsofa_options& o, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out) {
+int block_grid(const gsofa_options& o, int S, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out) {
     if (o.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, o.mem_budget_bytes);
     int occ = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(), blk_block_size(), 0) !=
+    if (!blk_slots_valid(S)) return GSOFA_ERR_INVALID;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(S), blk_block_size(), 0) !=
             hipSuccess ||
         occ <= 0 ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) {
@@ -458,10 +496,10 @@ int block_grid(const gsofa_options& o, uint64_t nchunks, uint32_t W, int64_t ava
     }
     uint32_t nb = (uint32_t)occ * (uint32_t)cus;
     if (o.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)o.grid_blocks);
-    if (o.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, o.chunk / kBlockSlots));
+    if (o.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, o.chunk / S));
     nb = (uint32_t)std::min<uint64_t>(nb, std::max<uint64_t>(nchunks, 1));
-    while (block_bytes(nb, W) > avail && nb > 1) nb = nb > 64 ? nb - nb / 8 : nb - 1;
-    if (block_bytes(nb, W) > avail) return GSOFA_ERR_BUDGET;
+    while (block_bytes(nb, W, S) > avail && nb > 1) nb = nb > 64 ? nb - nb / 8 : nb - 1;
+    if (block_bytes(nb, W, S) > avail) return GSOFA_ERR_BUDGET;
     *nb_out = nb;
     return GSOFA_OK;
 }
@@ -509,11 +547,14 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
         return sz.total_bytes;
     }
     const int32_t N = std::max(1, o.n_virtual_gpu);
-    const uint64_t nchunks = (share_count(A->n, N, 0) + kBlockSlots - 1) / kBlockSlots;
+    if (o.block_slots != 0 && !blk_slots_valid(o.block_slots)) return -1;
+    int S = 0;
+    if (choose_slots(o, 1, share_count(A->n, N, 0), &S) != GSOFA_OK) return -1;
+    const uint64_t nchunks = (share_count(A->n, N, 0) + S - 1) / S;
     const uint64_t fc = o.output_capacity > 0 ? (uint64_t)o.output_capacity : default_fill_cap(A->nnz);
     uint32_t nb = 0;
-    if (block_grid(o, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
-    return block_bytes(nb, (uint32_t)A->n) + block_fixed_bytes(A->n, A->nnz, 1, nchunks, fc);
+    if (block_grid(o, S, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
+    return block_bytes(nb, (uint32_t)A->n, S) + block_fixed_bytes(A->n, A->nnz, 1, nchunks, fc);
 }
 
 }  // extern "C"
@@ -539,7 +580,8 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         P->opt.mode = GSOFA_MODE_BLOCK;
     }
     if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
-    if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu) {
+    if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu ||
+        (P->opt.block_slots != 0 && !blk_slots_valid(P->opt.block_slots))) {
         delete P;
         return GSOFA_ERR_INVALID;
     }
@@ -597,14 +639,18 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     }
     int e = 0;
     if (P->mode == GSOFA_MODE_BLOCK) {
-        build_chunks(P);
-        rc = size_block_mode(P, std::max<int64_t>(avail, 1 << 20));
+        int S = 0;
+        const uint64_t nsrc = P->hybrid ? P->s_top : share_count(n, P->opt.n_virtual_gpu, 0);
+        rc = choose_slots(P->opt, n_mats, nsrc, &S);
+        P->S = (uint32_t)S;
+        if (!rc) build_chunks(P);
+        if (!rc) rc = size_block_mode(P, std::max<int64_t>(avail, 1 << 20));
         if (rc) {
             (void)hipStreamDestroy(P->stream);
             delete P;
             return rc;
         }
-        const size_t slots = (size_t)P->nblocks * kBlockSlots;
+        const size_t slots = (size_t)P->nblocks * (size_t)P->S;
         e |= dalloc(&P->d_lab, slots * (size_t)P->W);
         e |= dalloc(&P->d_mask, (size_t)P->nblocks * 2 * (size_t)P->W);
         e |= dalloc(&P->d_qgv, (size_t)P->nblocks * 2 * (size_t)P->W);
@@ -887,7 +933,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
                     HIPCHK(launch_traverse(tp, P->stream, P->coop));
                     advance_groups(P, gid, P->hC, (uint64_t)P->h_g_lo, (uint64_t)P->h_g_hi);
                 }
-                if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->blk_lds, P->stream));
+                if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->S, P->blk_lds, P->stream));
                 HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
                 continue;
             }

@@ -4,24 +4,27 @@
 // (with_space_optimization/symbfact.cuh:883-1082) and seeding (:639-721), with
 // an execution model built for CDNA4:
 //
-// * Each workgroup of a persistent, occupancy-bounded grid owns 32 source
-//   slots and runs a "mini-group" of 32 consecutive sources through all of
-//   their frontier levels by itself, synchronising only with __syncthreads.
-//   The next mini-group comes from one global counter.  There is no grid-wide
-//   barrier and no hot global counter on the level path.
-// * Labels are stored slot-interleaved, [vertex][32 slots]: the
-//   32 sources' max-id labels of one vertex share one 128-byte line.  Nearby
-//   sources reach mostly the same vertices in the same levels, so the frontier
-//   is kept per vertex with a 32-bit mask of the slots that must expand it
+// * Each workgroup (16 waves, one per CU) of a persistent grid owns SL source
+//   slots (32, 16 or 8; one kernel instance each) and runs a "mini-group" of
+//   SL consecutive sources through all of their frontier levels by itself,
+//   synchronising only with __syncthreads.  The next mini-group comes from
+//   one global counter.  There is no grid-wide barrier and no hot global
+//   counter on the level path.
+// * Labels are stored slot-interleaved, [vertex][SL slots]: the SL sources'
+//   max-id labels of one vertex share one line (128 bytes at 32 slots).
+//   Nearby sources reach mostly the same vertices in the same levels, so the
+//   frontier is kept per vertex with a mask of the slots that must expand it
 //   (deduplicated: a vertex enters the next queue when its mask leaves zero).
+// * Work is spread over lane groups of SL lanes (two per wave at 32 slots,
+//   eight at 8): lane j of a group always serves slot j.
 // * A level runs in rounds of two phases, and no atomic touches global memory
 //   on the level path (on this part a global atomic executes at the memory
 //   side, not in the L2):
-//     A. combine: each half-wave pops a frontier vertex v, computes lane j's
-//        relaxation cost max(v, label_j(v)) for the slots j of v's mask
+//     A. combine: each lane group pops frontier vertices v, computes lane
+//        j's relaxation cost max(v, label_j(v)) for the slots j of v's mask
 //        (symbfact.cuh:924), looks each neighbour m up in an LDS hash table
 //        of the round's distinct targets, and folds the cost into that
-//        target's 32 per-slot cells with an LDS atomic min;
+//        target's SL per-slot cells with an LDS atomic min;
 //     B. relax: every table entry (a distinct target m) is relaxed once per
 //        slot against its label exactly as the reference relaxes
 //        one edge (symbfact.cuh:989-1066), with the round's minimum cost.
@@ -47,7 +50,7 @@
 //       from stale to a current value; neighbours start touched.
 // * Sources are taken in decreasing order so the deepest traversals start
 //   first.
-// * Work is a table of chunks (up to 32 sources of one matrix).  A batch of
+// * Work is a table of chunks (up to SL sources of one matrix).  A batch of
 //   matrices is one block-diagonal matrix in which no traversal can leave its
 //   own matrix, so per-workgroup arrays are indexed relative to the chunk's
 //   matrix and sized by the largest matrix W, and labels hold
@@ -62,47 +65,59 @@ namespace {
 using namespace dev;
 
 #ifndef GSOFA_BB
-#define GSOFA_BB 512
-#endif
-constexpr int BB = GSOFA_BB;         // threads per workgroup
-constexpr int BW = BB / WAVE;        // waves per workgroup
-constexpr int NH = 2 * BW;           // half-waves
-constexpr int SL = kBlockSlots;      // slots per workgroup (lanes of a half-wave)
-#ifndef GSOFA_T
-#define GSOFA_T kBlkTable
-#endif
-constexpr int T = GSOFA_T;           // distinct targets per round
-#ifndef GSOFA_HB
-#define GSOFA_HB 10
-#endif
-constexpr int HB = GSOFA_HB;         // log2 of the hash slots
-constexpr int H = 1 << HB;           // hash slots (load factor <= T / H)
-constexpr uint32_t INF = 0xFFFFFFFFu;
-constexpr uint32_t NONE = 0xFFFFFFFFu;   // no table entry (edge not in this group)
-constexpr uint32_t FULL = 0xFFFFFFFEu;   // table full: retry the edge next round
-constexpr uint32_t HOLE = 0xFFFFFFFFu;   // ent_m of an index that lost its insert
-#ifndef GSOFA_KR
-#define GSOFA_KR 16
-#endif
-#ifndef GSOFA_KV
-#define GSOFA_KV 4
+#define GSOFA_BB 1024
 #endif
 #ifndef GSOFA_WPE
 #define GSOFA_WPE 4
 #endif
-constexpr int KR = GSOFA_KR;         // table entries relaxed together per half-wave (<= 32)
-constexpr int KV = GSOFA_KV;         // frontier vertices fetched together per half-wave (<= 32)
-static_assert(KR <= 32 && KV <= 32, "one entry / vertex per lane of a half-wave");
-static_assert(T <= H / 2, "hash load factor");
+constexpr int BB = GSOFA_BB;         // threads per workgroup
+constexpr int BW = BB / WAVE;        // waves per workgroup
+constexpr uint32_t INF = 0xFFFFFFFFu;
+constexpr uint32_t NONE = 0xFFFFFFFFu;   // no table entry (edge not in this group)
+constexpr uint32_t FULL = 0xFFFFFFFEu;   // table full: retry the edge next round
+constexpr uint32_t HOLE = 0xFFFFFFFFu;   // ent_m of an index that lost its insert
 
+// Shape of one instantiation, by slots per workgroup SL (= lanes per lane
+// group).  One workgroup of 16 waves holds a CU and nearly all of its 160 KB
+// of LDS, most of it the per-round target table: the more distinct targets a
+// round takes, the fewer rounds (and barriers) a level needs, and that is
+// worth more than a second, smaller workgroup per CU.  Fewer slots leave
+// room for more targets.
+//   T  distinct targets per round, HB log2 of the hash slots (load <= 1/2),
+//   KR table entries relaxed together per lane group (<= SL),
+//   KV frontier vertices fetched together per lane group (<= SL).
+template <int SL> struct Cfg;
+template <> struct Cfg<32> { static constexpr int T = 1024, HB = 11, KR = 16, KV = 4; };
+template <> struct Cfg<16> { static constexpr int T = 1792, HB = 12, KR = 16, KV = 4; };
+template <> struct Cfg<8> { static constexpr int T = 2048, HB = 12, KR = 8, KV = 4; };
+
+template <int SL>
+struct Shape {
+    static constexpr int T = Cfg<SL>::T;
+    static constexpr int HB = Cfg<SL>::HB;
+    static constexpr int H = 1 << HB;            // hash slots
+    static constexpr int KR = Cfg<SL>::KR;
+    static constexpr int KV = Cfg<SL>::KV;
+    static constexpr int GPW = WAVE / SL;        // lane groups per wave
+    static constexpr int NG = BW * GPW;          // lane groups per workgroup
+    static constexpr uint32_t GM = SL == 32 ? 0xFFFFFFFFu : ((1u << SL) - 1u);  // a group's lanes
+    static_assert(SL == 8 || SL == 16 || SL == 32, "slots per workgroup");
+    static_assert(KR <= SL && KV <= SL, "one entry / vertex per lane of a group");
+    static_assert(T <= H / 2, "hash load factor");
+    static_assert(T % (GPW * KR) == 0, "relax batches tile the table");
+    static_assert(T * SL * 4 + H * 8 + T * 4 + NG * SL * 4 + 1024 <= 160 * 1024, "LDS of one CU");
+};
+
+template <int SL>
 struct BlkLds {
-    uint32_t cell[T][SL];            // min cost per (round target, slot); INF = none
-    unsigned long long hkey[H];      // ((m + 1) << 32) | table index; 0 = empty
-    uint32_t ent_m[T];               // table index -> target (window-local) or HOLE
-    uint4 eidx[NH][8];               // table index of each lane's edge (32 per half-wave)
-    unsigned long long s_te[SL];     // per-slot TE of the current mini-group
-    unsigned long long s_reach[SL];  // per-slot |R_<(s)|
-    unsigned long long acc[4];       // workgroup totals: fills, TE, |R|, edge checks
+    using Z = Shape<SL>;
+    uint32_t cell[Z::T][SL];             // min cost per (round target, slot); INF = none
+    unsigned long long hkey[Z::H];       // ((m + 1) << 32) | table index; 0 = empty
+    uint32_t ent_m[Z::T];                // table index -> target (window-local) or HOLE
+    uint4 eidx[Z::NG][SL / 4];           // table index of each lane's edge (SL per group)
+    unsigned long long s_te[SL];         // per-slot TE of the current mini-group
+    unsigned long long s_reach[SL];      // per-slot |R_<(s)|
+    unsigned long long acc[4];           // workgroup totals: fills, TE, |R|, edge checks
     uint32_t src[SL];
     uint32_t qcnt[2];
     uint32_t cursor;
@@ -133,8 +148,8 @@ __device__ __forceinline__ const T& at(const T* base, uint32_t byte_off) {
     return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
-// Per-lane tallies of the current mini-group (lane j's te / reach / fills
-// belong to slot j; one source's counts fit 32 bits since nnz < 2^31).
+// Per-lane tallies of the current mini-group (lane j of a group: slot j's te /
+// reach / fills; one source's counts fit 32 bits since nnz < 2^31).
 struct TallyB {
     uint32_t fills, te, reach;
     unsigned long long checks;
@@ -143,20 +158,22 @@ struct TallyB {
 // Table index of target m in this round, inserting it if new; FULL when the
 // table has no room left.  Linear probing over H slots; an index allocated for
 // an insert that loses its race to an equal key becomes a HOLE entry.
-__device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
+template <int SL>
+__device__ __forceinline__ uint32_t tab_find(BlkLds<SL>& S, uint32_t m) {
+    using Z = Shape<SL>;
     const unsigned long long key = (unsigned long long)(m + 1u) << 32;
-    uint32_t s = (m * 2654435761u) >> (32 - HB);
+    uint32_t s = (m * 2654435761u) >> (32 - Z::HB);
     {
         const unsigned long long cur = S.hkey[s];  // usual case: present at its first slot
         if ((cur & 0xFFFFFFFF00000000ull) == key) return (uint32_t)cur;
     }
     uint32_t mine = NONE, res = FULL;
-    for (int probe = 0; probe < H; probe++) {
+    for (int probe = 0; probe < Z::H; probe++) {
         unsigned long long cur = S.hkey[s];
         if (cur == 0ull) {
             if (mine == NONE) {
                 mine = atomicAdd(&S.tcnt, 1u);
-                if (mine >= (uint32_t)T) {
+                if (mine >= (uint32_t)Z::T) {
                     mine = NONE;
                     break;
                 }
@@ -174,76 +191,84 @@ __device__ __forceinline__ uint32_t tab_find(BlkLds& S, uint32_t m) {
             res = (uint32_t)cur;
             break;
         }
-        s = (s + 1u) & (uint32_t)(H - 1);
+        s = (s + 1u) & (uint32_t)(Z::H - 1);
     }
     if (mine != NONE) S.ent_m[mine] = HOLE;
     return res;
 }
 
-// Lane u's value of x within the caller's half-wave: a swizzle broadcast
-// inside each group of 32 lanes (and-mask 0, or-mask u).  u must fold to a
-// constant (the swizzle pattern is an immediate); the source lane must be
-// active.
-template <int U>
-__device__ __forceinline__ uint32_t bcast32(uint32_t x) {
-    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, U << 5);
+// Lane u's value of x within the caller's lane group of SL lanes: a swizzle
+// broadcast inside each group of 32 lanes whose and-mask keeps the group's
+// lane bits above log2(SL) and whose or-mask is u.  u must fold to a constant
+// (the swizzle pattern is an immediate); the source lane must be active.
+template <int SL, int U>
+__device__ __forceinline__ uint32_t bcast_grp(uint32_t x) {
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, (0x1F & ~(SL - 1)) | (U << 5));
 }
-__device__ __forceinline__ uint32_t from_lane(uint32_t x, uint32_t u, uint32_t /*half*/) {
-    switch (u) {
-#define GSOFA_BC(k) \
-    case k: return bcast32<k>(x);
-        GSOFA_BC(0) GSOFA_BC(1) GSOFA_BC(2) GSOFA_BC(3) GSOFA_BC(4) GSOFA_BC(5) GSOFA_BC(6) GSOFA_BC(7)
-        GSOFA_BC(8) GSOFA_BC(9) GSOFA_BC(10) GSOFA_BC(11) GSOFA_BC(12) GSOFA_BC(13) GSOFA_BC(14) GSOFA_BC(15)
-        GSOFA_BC(16) GSOFA_BC(17) GSOFA_BC(18) GSOFA_BC(19) GSOFA_BC(20) GSOFA_BC(21) GSOFA_BC(22) GSOFA_BC(23)
-        GSOFA_BC(24) GSOFA_BC(25) GSOFA_BC(26) GSOFA_BC(27) GSOFA_BC(28) GSOFA_BC(29) GSOFA_BC(30) GSOFA_BC(31)
-#undef GSOFA_BC
-        default: return 0u;
+template <int SL, int U = 0>
+__device__ __forceinline__ uint32_t from_lane(uint32_t x, uint32_t u) {
+    if constexpr (U == SL) {
+        return 0u;
+    } else {
+        if (u == (uint32_t)U) return bcast_grp<SL, U>(x);
+        return from_lane<SL, U + 1>(x, u);
     }
 }
 
-// Frontier vertices held by a half-wave across rounds.  Lane k of the half
+// The calling lane group's bits of a wave ballot (bit k: lane k of the group).
+template <int SL>
+__device__ __forceinline__ uint32_t grp_ballot(bool x, uint32_t grp) {
+    return (uint32_t)(__ballot(x) >> (SL * grp)) & Shape<SL>::GM;
+}
+
+// Frontier vertices held by a lane group across rounds.  Lane k of the group
 // keeps vertex k's CSR cursor (pos, end) and its pending-edge mask for the
-// current 32-edge group; lane j keeps slot j's cost of every vertex and the
+// current SL-edge group; lane j keeps slot j's cost of every vertex and the
 // j-th edge target of its current group.
-struct HalfVtx {
-    bool held;           // half-uniform: a batch is in progress
-    bool done;           // half-uniform: the queue is exhausted for this level
+template <int KV>
+struct GrpVtx {
+    bool held;           // group-uniform: a batch is in progress
+    bool done;           // group-uniform: the queue is exhausted for this level
     int32_t pos, end;    // lane k: vertex k's current group start, row end
     uint32_t pend;       // lane k: vertex k's edges of the group not yet folded
     uint32_t cost[KV];   // lane j: max(v_u + off, label_j(v_u)), INF if j not in v_u's mask
     uint32_t civ[KV];    // lane j: target of edge j of v_u's current group (window-local)
 };
 
+template <int SL>
 __device__ __forceinline__ uint32_t group_mask(int32_t left) {
-    return left >= 32 ? 0xFFFFFFFFu : (left <= 0 ? 0u : ((1u << left) - 1u));
+    return left >= SL ? Shape<SL>::GM : (left <= 0 ? 0u : ((1u << left) - 1u));
 }
 
-// Phase A: pop frontier vertices (KV per half-wave) and fold their edges'
+// Phase A: pop frontier vertices (KV per lane group) and fold their edges'
 // costs into the table until the queue is exhausted or the table is full.
-// Wave-uniform control flow; the two halves carry independent batches.
-__device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur, uint32_t size,
-                              TallyB& t) {
+// Wave-uniform control flow; the lane groups carry independent batches.
+template <int SL>
+__device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>& hv, uint32_t cur,
+                              uint32_t size, TallyB& t) {
+    using Z = Shape<SL>;
+    constexpr int KV = Z::KV;
     const BlkParams& p = *B.p;
     const uint32_t lane = __lane_id();
-    const uint32_t half = lane >> 5;
-    const uint32_t j = lane & 31u;
-    const uint32_t h = (threadIdx.x / WAVE) * 2 + half;
+    const uint32_t grp = lane / SL;
+    const uint32_t j = lane % SL;
+    const uint32_t h = (threadIdx.x / WAVE) * Z::GPW + grp;
     uint32_t* maskc = B.mask0 + (uint64_t)cur * B.W;
     const uint4* qc = B.q0 + (uint64_t)cur * B.W;
     if (hv.held) {  // resuming after a full table: reload the pending groups' targets
 #pragma unroll
         for (int u = 0; u < KV; u++) {
-            const uint32_t pu = from_lane(hv.pend, u, half);
-            const int32_t pos = (int32_t)from_lane((uint32_t)hv.pos, u, half);
+            const uint32_t pu = from_lane<SL>(hv.pend, u);
+            const int32_t pos = (int32_t)from_lane<SL>((uint32_t)hv.pos, u);
             hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
         }
     }
     for (;;) {
-        // -- take a new batch where a half has none
-        const bool want = !hv.held && !hv.done && S.tcnt < (uint32_t)T;
+        // -- take a new batch where a group has none
+        const bool want = !hv.held && !hv.done && S.tcnt < (uint32_t)Z::T;
         uint32_t i0 = 0;
         if (j == 0 && want) i0 = atomicAdd(&S.cursor, (uint32_t)KV);
-        i0 = from_lane(i0, 0, half);
+        i0 = from_lane<SL>(i0, 0);
         if (want && i0 >= size) hv.done = true;
         const bool fresh = want && i0 < size;
         if (__ballot(fresh)) {
@@ -256,15 +281,15 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
             if (fresh) {
                 hv.pos = (int32_t)q.y;
                 hv.end = (int32_t)q.z;
-                hv.pend = group_mask((int32_t)(q.z - q.y));
+                hv.pend = group_mask<SL>((int32_t)(q.z - q.y));
             }
             // every vertex's label line and first edge group, issued together
             uint32_t lv[KV];
 #pragma unroll
             for (int u = 0; u < KV; u++) {
-                const uint32_t vu = from_lane(q.x, u, half);
-                const uint32_t pu = from_lane(hv.pend, u, half);
-                const int32_t pos = (int32_t)from_lane((uint32_t)hv.pos, u, half);
+                const uint32_t vu = from_lane<SL>(q.x, u);
+                const uint32_t pu = from_lane<SL>(hv.pend, u);
+                const int32_t pos = (int32_t)from_lane<SL>((uint32_t)hv.pos, u);
                 lv[u] = 0u;
                 if (fresh && vu != NONE) lv[u] = at(B.lab0, (vu * SL + j) << 2);
                 if (fresh) hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
@@ -275,26 +300,26 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
             }
 #pragma unroll
             for (int u = 0; u < KV; u++) {
-                const uint32_t vu = from_lane(q.x, u, half);
-                const uint32_t Mu = from_lane(M, u, half);
+                const uint32_t vu = from_lane<SL>(q.x, u);
+                const uint32_t Mu = from_lane<SL>(M, u);
                 const uint32_t vo = vu + B.off;
                 if (fresh)  // symbfact.cuh:924
                     hv.cost[u] = ((Mu >> j) & 1u) && vu != NONE ? (vo > lv[u] ? vo : lv[u]) : INF;
             }
             if (fresh) hv.held = true;
         }
-        if (!__ballot(hv.held)) return;  // nothing left for either half
+        if (!__ballot(hv.held)) return;  // nothing left for any group
         // -- fold the held vertices' edges, vertex by vertex
         bool full = false;
 #pragma unroll
         for (int u = 0; u < KV; u++) {
-            uint32_t pu = hv.held ? from_lane(hv.pend, u, half) : 0u;
-            while (pu) {  // half-uniform
+            uint32_t pu = hv.held ? from_lane<SL>(hv.pend, u) : 0u;
+            while (pu) {  // group-uniform
                 const bool mine = (pu >> j) & 1u;
                 uint32_t idx = NONE;
-                if (mine) idx = tab_find(S, hv.civ[u]);
+                if (mine) idx = tab_find<SL>(S, hv.civ[u]);
                 reinterpret_cast<uint32_t*>(S.eidx[h])[j] = idx;
-                const uint32_t okm = (uint32_t)(__ballot(mine && idx != FULL) >> (32u * half));
+                const uint32_t okm = grp_ballot<SL>(mine && idx != FULL, grp);
                 wave_sync();
                 if (hv.cost[u] != INF && okm) {
                     const uint32_t cu = hv.cost[u];
@@ -332,11 +357,11 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
                     break;
                 }
                 // next group of a long row (rare): load its targets now
-                int32_t pos = (int32_t)from_lane((uint32_t)hv.pos, u, half);
-                const int32_t end = (int32_t)from_lane((uint32_t)hv.end, u, half);
-                if (pos + 32 < end) {
-                    pos += 32;
-                    pu = group_mask(end - pos);
+                int32_t pos = (int32_t)from_lane<SL>((uint32_t)hv.pos, u);
+                const int32_t end = (int32_t)from_lane<SL>((uint32_t)hv.end, u);
+                if (pos + SL < end) {
+                    pos += SL;
+                    pu = group_mask<SL>(end - pos);
                     if (j == (uint32_t)u) hv.pos = pos;
                     hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
                 }
@@ -348,24 +373,27 @@ __device__ void combine_phase(const Blk& B, BlkLds& S, HalfVtx& hv, uint32_t cur
     }
 }
 
-// Phase B: relax every table entry once per slot.  Each half-wave takes 32
+// Phase B: relax every table entry once per slot.  Each lane group takes KR
 // entries: lane k loads entry k's CSR range and next-level mask, then the
-// half issues all 32 label-line loads before using any, so a batch costs one
-// memory round trip.  Wave-uniform control flow.
-__device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_src, uint32_t my_srcg,
+// group issues all KR label-line loads before using any, so a batch costs
+// one memory round trip.  Wave-uniform control flow.
+template <int SL>
+__device__ void relax_phase(const Blk& B, BlkLds<SL>& S, uint32_t nbuf, uint32_t my_src, uint32_t my_srcg,
                             TallyB& t) {
+    using Z = Shape<SL>;
+    constexpr int KR = Z::KR;
     const BlkParams& p = *B.p;
     const uint32_t lane = __lane_id();
-    const uint32_t half = lane >> 5;
-    const uint32_t j = lane & 31u;
+    const uint32_t grp = lane / SL;
+    const uint32_t j = lane % SL;
     const uint32_t w = threadIdx.x / WAVE;
     uint32_t* maskn = B.mask0 + (uint64_t)nbuf * B.W;
     uint4* qn = B.q0 + (uint64_t)nbuf * B.W;
     const uint32_t tc = S.tcnt;
-    const uint32_t ntab = tc < (uint32_t)T ? tc : (uint32_t)T;
+    const uint32_t ntab = tc < (uint32_t)Z::T ? tc : (uint32_t)Z::T;
     const uint32_t off = B.off;
-    for (uint32_t base = w * (2 * KR); base < ntab; base += BW * (2 * KR)) {
-        const uint32_t eb = base + half * KR;
+    for (uint32_t base = w * (Z::GPW * KR); base < ntab; base += BW * (Z::GPW * KR)) {
+        const uint32_t eb = base + grp * KR;
         // lane k: entry eb + k
         const uint32_t ek = eb + j;
         uint32_t mk = HOLE;
@@ -384,7 +412,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
         uint32_t lm[KR];
 #pragma unroll
         for (int u = 0; u < KR; u++) {
-            const uint32_t mu = from_lane(mk, u, half);
+            const uint32_t mu = from_lane<SL>(mk, u);
             lm[u] = 0u;
             if (mu != HOLE) lm[u] = at(B.lab0, (mu * SL + j) << 2);
         }
@@ -393,8 +421,8 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
 #pragma unroll
         for (int u = 0; u < KR; u++) {
             // broadcasts while every lane is active
-            const uint32_t mu = from_lane(mk, u, half);
-            const uint32_t du = from_lane(dd, u, half);
+            const uint32_t mu = from_lane<SL>(mk, u);
+            const uint32_t du = from_lane<SL>(dd, u);
             // hole and unused cells are INF: no entry check needed
             const uint32_t c = S.cell[eb + u][j];
             S.cell[eb + u][j] = INF;  // ready for the next round
@@ -414,7 +442,7 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
                     t.te += fr ? du : 0u;
                 }
             }
-            const uint32_t pb = (uint32_t)(__ballot(push) >> (32u * half));
+            const uint32_t pb = grp_ballot<SL>(push, grp);
             if (j == (uint32_t)u) mybits = pb;
             if (fill) fillbits |= 1u << u;
         }
@@ -447,12 +475,14 @@ __device__ void relax_phase(const Blk& B, BlkLds& S, uint32_t nbuf, uint32_t my_
     }
 }
 
+template <int SL>
 __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, GSOFA_WPE))) void k_traverse_blk(BlkParams p) {
-    __shared__ BlkLds S;
+    using Z = Shape<SL>;
+    __shared__ BlkLds<SL> S;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / WAVE;
-    const uint32_t j = lane & 31u;
+    const uint32_t j = lane % SL;
     const uint32_t W = p.W;
     const uint32_t loops = 0xFFFFFFFFu / W;
     Blk B;
@@ -465,8 +495,8 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
     unsigned long long levels = 0;
-    for (uint32_t i = tid; i < (uint32_t)(T * SL); i += BB) (&S.cell[0][0])[i] = INF;
-    for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
+    for (uint32_t i = tid; i < (uint32_t)(Z::T * SL); i += BB) (&S.cell[0][0])[i] = INF;
+    for (uint32_t i = tid; i < (uint32_t)Z::H; i += BB) S.hkey[i] = 0ull;
     if (tid < 4) S.acc[tid] = 0;
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
@@ -478,7 +508,7 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
         const uint32_t ck = S.chunk;
         if (ck >= p.n_chunks) break;
         const BlkChunk C = p.chunks[ck];
-        const uint32_t nsrc = C.count;
+        const uint32_t nsrc = C.count;  // <= SL (the host's chunking)
         B.wb = p.mat_base[C.mat];
         if (tid < (uint32_t)SL) {
             S.src[tid] = tid < nsrc ? C.first - tid * C.stride : 0u;
@@ -548,20 +578,20 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
             __syncthreads();  // every thread has the size before it can change
             if (size == 0) break;
             const uint32_t nbuf = cur ^ 1u;
-            HalfVtx hv;
+            GrpVtx<Z::KV> hv;
             hv.held = false;
             hv.done = false;
             hv.pend = 0u;
             hv.pos = hv.end = 0;
             for (;;) {  // rounds
-                combine_phase(B, S, hv, cur, size, t);
+                combine_phase<SL>(B, S, hv, cur, size, t);
                 if (hv.held && j == 0) atomicOr(&S.held, 1u);
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
                 const bool more = S.held != 0u || S.cursor < size;
-                relax_phase(B, S, nbuf, my_src, my_srcg, t);
+                relax_phase<SL>(B, S, nbuf, my_src, my_srcg, t);
                 __syncthreads();
-                for (uint32_t i = tid; i < (uint32_t)H; i += BB) S.hkey[i] = 0ull;
+                for (uint32_t i = tid; i < (uint32_t)Z::H; i += BB) S.hkey[i] = 0ull;
                 if (tid == 0) {
                     S.tcnt = 0;
                     S.held = 0;
@@ -607,11 +637,23 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
 
 }  // namespace
 
+bool blk_slots_valid(int slots) { return slots == 32 || slots == 16 || slots == 8; }
 int blk_block_size() { return BB; }
-const void* blk_kernel_ptr() { return reinterpret_cast<const void*>(&k_traverse_blk); }
+const void* blk_kernel_ptr(int slots) {
+    switch (slots) {
+        case 16: return reinterpret_cast<const void*>(&k_traverse_blk<16>);
+        case 8: return reinterpret_cast<const void*>(&k_traverse_blk<8>);
+        default: return reinterpret_cast<const void*>(&k_traverse_blk<32>);
+    }
+}
 
-hipError_t launch_traverse_blk(const BlkParams& p, size_t lds_bytes, hipStream_t s) {
-    hipLaunchKernelGGL(k_traverse_blk, dim3(p.nblocks), dim3(BB), lds_bytes, s, p);
+hipError_t launch_traverse_blk(const BlkParams& p, int slots, size_t lds_bytes, hipStream_t s) {
+    switch (slots) {
+        case 32: hipLaunchKernelGGL(k_traverse_blk<32>, dim3(p.nblocks), dim3(BB), lds_bytes, s, p); break;
+        case 16: hipLaunchKernelGGL(k_traverse_blk<16>, dim3(p.nblocks), dim3(BB), lds_bytes, s, p); break;
+        case 8: hipLaunchKernelGGL(k_traverse_blk<8>, dim3(p.nblocks), dim3(BB), lds_bytes, s, p); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

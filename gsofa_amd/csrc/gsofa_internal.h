@@ -35,12 +35,12 @@ struct Ctrl {
     unsigned long long snap_orph[2];
 };
 
-constexpr int kBlockSlots = 32;      // source slots per workgroup in block mode
-constexpr int kBlkTable = 512;       // distinct relaxation targets per round (LDS table)
+constexpr int kBlockSlots = 32;      // largest source-slot count per workgroup in block mode
+constexpr int kBatchSlots = 16;      // slots per workgroup for batches of matrices by default
 
-// Block-resident mode: each workgroup owns kBlockSlots source slots.
+// Block-resident mode: each workgroup owns S source slots (32, 16 or 8).
 // Work comes as a table of chunks, processed in table order (decreasing
-// sources): chunk c covers sources first, first - stride, ... (count <= 32) of
+// sources): chunk c covers sources first, first - stride, ... (count <= S) of
 // matrix `mat`, whose vertices are [mat_base[mat], mat_base[mat+1]) in the
 // global numbering of a block-diagonal batch (a single matrix is mat 0).
 struct BlkChunk {
@@ -59,7 +59,7 @@ struct BlkParams {
     uint32_t n_chunks;
     const uint32_t* mat_base;  // n_mats + 1 vertex offsets
     uint32_t nblocks;
-    uint32_t* lab;         // nblocks * W * kBlockSlots labels, [vertex][slot]
+    uint32_t* lab;         // nblocks * W * S labels, [vertex][slot]
     uint32_t* mask;        // nblocks * 2 * W frontier slot masks
     uint4* qg;             // nblocks * 2 * W frontier entries (v, row begin, row end, -)
     uint64_t* fills;
@@ -70,13 +70,15 @@ struct BlkParams {
     unsigned long long* row_reach;  // n: |R_<(s)| of each processed source row
 };
 int blk_block_size();
+bool blk_slots_valid(int slots);
+int choose_slots(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* slots);
 // Block-mode sizing shared by plan creation, footprints and the server.
-int64_t block_bytes(uint32_t nb, uint32_t W);
+int64_t block_bytes(uint32_t nb, uint32_t W, int slots);
 int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap);
 uint64_t default_fill_cap(int64_t nnz);
-int block_grid(const gsofa_options& o, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
-const void* blk_kernel_ptr();
-hipError_t launch_traverse_blk(const BlkParams& p, size_t lds_bytes, hipStream_t s);
+int block_grid(const gsofa_options& o, int slots, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
+const void* blk_kernel_ptr(int slots);
+hipError_t launch_traverse_blk(const BlkParams& p, int slots, size_t lds_bytes, hipStream_t s);
 
 struct TravParams {
     const int32_t* rp;     // n+1 packed row pointers

@@ -73,7 +73,7 @@ int64_t fixed_bytes(const std::vector<Request*>& reqs) {
     for (const Request* r : reqs) {
         n += r->n;
         nnz += r->nnz;
-        nchunks += ((uint64_t)r->n + gsofa::kBlockSlots - 1) / gsofa::kBlockSlots;
+        nchunks += ((uint64_t)r->n + gsofa::kBatchSlots - 1) / gsofa::kBatchSlots;
     }
     return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, gsofa::default_fill_cap(nnz));
 }
@@ -86,7 +86,7 @@ uint32_t largest(const std::vector<Request*>& reqs) {
 
 // Traversal bytes left for the batch within one share (<= 0: does not fit).
 int64_t trav_room(const gsofa_server* S, const std::vector<Request*>& reqs) {
-    return S->share - fixed_bytes(reqs) - gsofa::block_bytes(1, largest(reqs));
+    return S->share - fixed_bytes(reqs) - gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots);
 }
 
 void finish(gsofa_server* S, Request* r, int status) {
@@ -107,6 +107,7 @@ void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id) 
     o.device = S->opt.device;
     o.max_supernode_size = S->opt.max_supernode_size;
     o.flags = S->opt.flags;
+    o.block_slots = gsofa::kBatchSlots;  // also for a batch of one: the sizing above assumes it
     // the traversal arrays get what the fixed part leaves of the share
     o.mem_budget_bytes = std::max<int64_t>(S->share - fixed_bytes(batch), 1);
     gsofa_plan* P = nullptr;

@@ -1,7 +1,8 @@
 """Autotuner for the runtime knobs of the traversal.
 
 The knobs that remain at run time are the execution mode, the persistent grid
-size and the number of top sources run as whole-device groups (block mode),
+size, the number of top sources run as whole-device groups and the sources
+per workgroup (block mode),
 and the concurrent-source chunk and wave/block expansion threshold (grid
 mode).  Results do not depend on any knob (the parity tests
 check this), only time does, so the tuner compares traversal times alone,
@@ -29,7 +30,8 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "gsofa_amd", "tu
 BLOCK_CANDIDATES = [
     {"mode": "block"},
     {"mode": "block", "grid_blocks": 384},
-] + [{"mode": "block", "top_sources": k} for k in (2048, 8192, 32768)]
+] + [{"mode": "block", "top_sources": k} for k in (2048, 8192, 32768)] + [
+    {"mode": "block", "block_slots": s} for s in (16, 8)]
 GRID_CANDIDATES = [
     {"mode": "grid", "chunk": c, "wave_block_threshold": th}
     for c in (2048, 8192) for th in (256, 4096)

@@ -86,6 +86,8 @@ typedef struct gsofa_options {
     int32_t top_sources;          /* block mode, one matrix, one share: the top sources     */
                                   /* (rounded up to whole groups) first run as grid-mode    */
                                   /* groups (GSOFA_MODE_HYBRID); 0 = none                   */
+    int32_t block_slots;          /* block mode: sources per workgroup, 32, 16 or 8; 0 = 32 */
+                                  /* (fewer slots, larger per-round target table)            */
 } gsofa_options;
 
 /* Execution modes of the traversal (identical results):

@@ -344,6 +344,24 @@ def test_server_argument_checks():
         assert ei.value.code == -6
 
 
+def test_block_slots_argument_check():
+    """Slots per workgroup other than 32, 16 or 8 are refused before any
+    device work (plan creation and the footprint query)."""
+    import ctypes as C
+    import gsofa_amd as g
+    from gsofa_amd import _lib
+    A = g.CsrMatrix.from_rows([[0, 1], [0, 1, 2], [1, 2]])
+    for bad in (1, 12, 64, -4):
+        with pytest.raises(g.GsofaError) as ei:
+            g.symbfact(A, block_slots=bad)
+        assert ei.value.code == -1, bad
+        o = g._options(block_slots=bad)
+        m = A.to_c()
+        assert _lib.lib().gsofa_plan_footprint(C.byref(m), C.byref(o)) == -1, bad
+    assert g._options(block_slots=8).block_slots == 8
+    assert g._options().block_slots == 0
+
+
 def test_tune_signature_and_cache(tmp_path):
     """Autotuner host logic: signatures bucket sizes, the cache round-trips."""
     import gsofa_amd as g
