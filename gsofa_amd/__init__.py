@@ -113,6 +113,53 @@ def read_ascii(colbeg_path, colend_path, csr_path):
     return _from_csr(c)
 
 
+def read_binary(path):
+    """Binary triple (include/gsofa.h, gsofa_read_binary) as a CsrMatrix."""
+    c = _lib.Csr()
+    _check(_lib.lib().gsofa_read_binary(path.encode(), C.byref(c)), "read_binary")
+    return _from_csr(c)
+
+
+def write_binary(A, path):
+    m = A.to_c()
+    _check(_lib.lib().gsofa_write_binary(C.byref(m), path.encode()), "write_binary")
+
+
+class MappedMatrix:
+    """A binary triple mapped read-only (gsofa_map_binary).  ``matrix`` is a
+    CsrMatrix whose arrays view the mapping: valid until close()."""
+
+    def __init__(self, path):
+        h = C.c_void_p()
+        v = _lib.Matrix()
+        _check(_lib.lib().gsofa_map_binary(path.encode(), C.byref(h), C.byref(v)), "map_binary")
+        self._h = h
+        n, nnz = int(v.n), int(v.nnz)
+
+        def view(ptr, cnt):
+            if cnt == 0:
+                return np.zeros(0, np.int32)
+            a = np.ctypeslib.as_array(C.cast(ptr, C.POINTER(C.c_int32)), shape=(cnt,))
+            a.flags.writeable = False
+            return a
+        self.matrix = CsrMatrix(n, view(v.colbeg, n), view(v.colend, n), view(v.csr, nnz))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.matrix = None
+            _lib.lib().gsofa_unmap_binary(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        self.close()
+
+
 def write_ascii(A, colbeg_path, colend_path, csr_path):
     m = A.to_c()
     _check(_lib.lib().gsofa_write_ascii(C.byref(m), colbeg_path.encode(), colend_path.encode(),
@@ -186,6 +233,29 @@ class SymbfactResult:
             self.supno = None
 
 
+def _carr(ptr, cnt, ct, dt):
+    if cnt <= 0:
+        return np.zeros(0, dt)
+    return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(cnt,)).copy()
+
+
+class SupernodalResult:
+    """Supernodal hand-off (include/gsofa.h, gsofa_supernodal): supernode
+    starts and ids, per-supernode subscripts, and L and U by columns."""
+
+    def __init__(self, h):
+        n, ns = int(h.n), int(h.n_super)
+        self.n, self.n_super = n, ns
+        self.xsup = _carr(h.xsup, ns + 1, C.c_int32, np.int32)
+        self.supno = _carr(h.supno, n, C.c_int32, np.int32)
+        self.xsub = _carr(h.xsub, ns + 1, C.c_int64, np.int64)
+        self.sub = _carr(h.sub, int(self.xsub[-1]), C.c_int32, np.int32)
+        self.xlcol = _carr(h.xlcol, n + 1, C.c_int64, np.int64)
+        self.lrow = _carr(h.lrow, int(self.xlcol[-1]), C.c_int32, np.int32)
+        self.xucol = _carr(h.xucol, n + 1, C.c_int64, np.int64)
+        self.urow = _carr(h.urow, int(self.xucol[-1]), C.c_int32, np.int32)
+
+
 def symbfact(A, **kw):
     """One-shot symbolic factorization of ``A`` (a CsrMatrix) on the GPU."""
     m = A.to_c()
@@ -242,6 +312,16 @@ class Plan:
             return SymbfactResult(r)
         finally:
             _lib.lib().gsofa_result_free(C.byref(r))
+
+    def fetch_supernodal(self):
+        """SuperLU_DIST-style hand-off of the last execute (single matrix,
+        every share, structure built)."""
+        h = _lib.Supernodal()
+        _check(_lib.lib().gsofa_plan_fetch_supernodal(self._h, C.byref(h)), "plan_fetch_supernodal")
+        try:
+            return SupernodalResult(h)
+        finally:
+            _lib.lib().gsofa_supernodal_free(C.byref(h))
 
     def close(self):
         if getattr(self, "_h", None):

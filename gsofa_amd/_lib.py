@@ -22,6 +22,8 @@ EXPORTS = [
     "gsofa_gen_unit", "gsofa_batch_create", "gsofa_plan_matrix_count", "gsofa_plan_fetch_matrix",
     "gsofa_server_options_default", "gsofa_server_create", "gsofa_server_submit",
     "gsofa_server_poll", "gsofa_server_wait", "gsofa_server_destroy",
+    "gsofa_plan_fetch_supernodal", "gsofa_supernodal_free",
+    "gsofa_write_binary", "gsofa_read_binary", "gsofa_map_binary", "gsofa_unmap_binary",
 ]
 GSOFA_PENDING = 1
 
@@ -57,6 +59,12 @@ class Result(C.Structure):
                 ("grid_blocks", C.c_int32),
                 ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64),
                 ("row_te", C.c_void_p), ("row_reach", C.c_void_p)]
+
+
+class Supernodal(C.Structure):
+    _fields_ = [("n", C.c_int32), ("n_super", C.c_int32), ("xsup", C.c_void_p), ("supno", C.c_void_p),
+                ("xsub", C.c_void_p), ("sub", C.c_void_p), ("xlcol", C.c_void_p), ("lrow", C.c_void_p),
+                ("xucol", C.c_void_p), ("urow", C.c_void_p)]
 
 
 class ServerOptions(C.Structure):
@@ -102,8 +110,18 @@ def lib():
     L.gsofa_plan_matrix_count.restype = C.c_int32
     L.gsofa_plan_fetch_matrix.argtypes = [C.c_void_p, C.c_int32, P(Result)]
     L.gsofa_plan_fetch_matrix.restype = C.c_int
+    L.gsofa_plan_fetch_supernodal.argtypes = [C.c_void_p, P(Supernodal)]
+    L.gsofa_plan_fetch_supernodal.restype = C.c_int
+    L.gsofa_supernodal_free.argtypes = [P(Supernodal)]
     L.gsofa_plan_footprint.argtypes = [P(Matrix), P(Options)]
     L.gsofa_plan_footprint.restype = C.c_int64
+    L.gsofa_write_binary.argtypes = [P(Matrix), C.c_char_p]
+    L.gsofa_write_binary.restype = C.c_int
+    L.gsofa_read_binary.argtypes = [C.c_char_p, P(Csr)]
+    L.gsofa_read_binary.restype = C.c_int
+    L.gsofa_map_binary.argtypes = [C.c_char_p, P(C.c_void_p), P(Matrix)]
+    L.gsofa_map_binary.restype = C.c_int
+    L.gsofa_unmap_binary.argtypes = [C.c_void_p]
     L.gsofa_read_ascii.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, P(Csr)]
     L.gsofa_read_ascii.restype = C.c_int
     L.gsofa_write_ascii.argtypes = [P(Matrix), C.c_char_p, C.c_char_p, C.c_char_p]

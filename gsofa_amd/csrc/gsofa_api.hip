@@ -143,6 +143,25 @@ int dalloc(T** p, size_t count) {
     return GSOFA_OK;
 }
 
+// Device allocations released together when the holder goes out of scope.
+struct DevScratch {
+    std::vector<void*> p;
+    ~DevScratch() {
+        for (void* q : p) (void)hipFree(q);
+    }
+    template <class T>
+    int get(T** out, size_t count) {
+        void* q = nullptr;
+        if (hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1)) != hipSuccess) {
+            (void)hipGetLastError();
+            return GSOFA_ERR_ALLOC;
+        }
+        p.push_back(q);
+        *out = (T*)q;
+        return GSOFA_OK;
+    }
+};
+
 template <class T>
 void dfree(T*& p) {
     if (p) (void)hipFree(p);
@@ -1190,6 +1209,88 @@ void gsofa_plan_destroy(gsofa_plan* P) {
     release_device(P);
     (void)hipStreamDestroy(P->stream);
     delete P;
+}
+
+void gsofa_supernodal_free(gsofa_supernodal* h) {
+    if (!h) return;
+    free(h->xsup);
+    free(h->supno);
+    free(h->xsub);
+    free(h->sub);
+    free(h->xlcol);
+    free(h->lrow);
+    free(h->xucol);
+    free(h->urow);
+    memset(h, 0, sizeof(*h));
+}
+
+int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
+    if (!P || !h) return GSOFA_ERR_INVALID;
+    memset(h, 0, sizeof(*h));
+    if (!P->executed) return GSOFA_ERR_STATE;
+    if (P->n_mats != 1 || !P->structure || (!P->supernodes && P->n > 0)) return GSOFA_ERR_INVALID;
+    const int32_t n = P->n;
+    h->n = n;
+    int32_t ns = 0;
+    HIPCHK(hipSetDevice(P->device));
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+    }
+    h->n_super = ns;
+    DevScratch T;  // device scratch for this call only
+    int64_t *d_xl = nullptr, *d_xu = nullptr, *d_xs = nullptr;
+    int32_t *d_lrow = nullptr, *d_urow = nullptr;
+    if (T.get(&d_xl, (size_t)n + 1) || T.get(&d_xu, (size_t)n + 1) || T.get(&d_xs, (size_t)ns + 1) ||
+        T.get(&d_lrow, (size_t)P->nnzL) || T.get(&d_urow, (size_t)P->nnzU))
+        return GSOFA_ERR_ALLOC;
+    // L and U by columns; the staging arrays of the structure pass are free now
+    HIPCHK(transpose_structure(P->d_Lrp, P->d_Lcol, P->nnzL, n, P->d_cntL, d_xl, P->d_scan, P->d_Ltmp, d_lrow,
+                               P->stream));
+    HIPCHK(transpose_structure(P->d_Urp, P->d_Ucol, P->nnzU, n, P->d_cntU, d_xu, P->d_scan, P->d_Utmp, d_urow,
+                               P->stream));
+    // supernode subscripts
+    int64_t nsub = 0;
+    int32_t* d_sub = nullptr;
+    if (ns > 0) {
+        HIPCHK(supernode_sub_count(P->d_sstart, ns, P->d_Urp, P->d_cntL, P->stream));
+        HIPCHK(exclusive_scan_u32_to_i64(P->d_cntL, d_xs, ns, P->d_scan, P->stream));
+        HIPCHK(hipMemcpyAsync(&nsub, d_xs + ns, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        if (T.get(&d_sub, (size_t)nsub)) return GSOFA_ERR_ALLOC;
+        HIPCHK(supernode_sub_write(P->d_sstart, ns, P->d_Urp, P->d_Ucol, d_xs, d_sub, P->stream));
+    }
+    h->xsup = (int32_t*)malloc(sizeof(int32_t) * ((size_t)ns + 1));
+    h->supno = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>((size_t)n, 1));
+    h->xsub = (int64_t*)calloc((size_t)ns + 1, sizeof(int64_t));
+    h->sub = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>((size_t)nsub, 1));
+    h->xlcol = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    h->lrow = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>((size_t)P->nnzL, 1));
+    h->xucol = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+    h->urow = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>((size_t)P->nnzU, 1));
+    if (!h->xsup || !h->supno || !h->xsub || !h->sub || !h->xlcol || !h->lrow || !h->xucol || !h->urow) {
+        gsofa_supernodal_free(h);
+        return GSOFA_ERR_ALLOC;
+    }
+    h->xsup[0] = 0;
+    if (n > 0) {
+        HIPCHK(hipMemcpyAsync(h->xsup, P->d_sstart, sizeof(int32_t) * ((size_t)ns + 1), hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipMemcpyAsync(h->supno, P->d_supno, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipMemcpyAsync(h->xlcol, d_xl, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(h->xucol, d_xu, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, P->stream));
+    }
+    if (P->nnzL)
+        HIPCHK(hipMemcpyAsync(h->lrow, d_lrow, sizeof(int32_t) * (size_t)P->nnzL, hipMemcpyDeviceToHost, P->stream));
+    if (P->nnzU)
+        HIPCHK(hipMemcpyAsync(h->urow, d_urow, sizeof(int32_t) * (size_t)P->nnzU, hipMemcpyDeviceToHost, P->stream));
+    if (ns > 0) {
+        HIPCHK(hipMemcpyAsync(h->xsub, d_xs, sizeof(int64_t) * ((size_t)ns + 1), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(h->sub, d_sub, sizeof(int32_t) * (size_t)nsub, hipMemcpyDeviceToHost, P->stream));
+    }
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return GSOFA_OK;
 }
 
 int gsofa_symbfact(const gsofa_matrix* A, const gsofa_options* opt, gsofa_result* out) {

@@ -12,7 +12,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
+#include <string>
 #include <vector>
+
+#include <fcntl.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "gsofa.h"
 
@@ -244,48 +251,151 @@ double gsofa_gen_unit(uint64_t seed, uint64_t purpose, uint64_t index) {
 }
 
 // ---------------------------------------------------------------------------
-// Reference ASCII format (with_space_optimization/symbfact.cuh:1591-1640):
-// csr file = nnz, n, then nnz indices; colbeg/colend = n integers each,
-// whitespace separated.  Only n values are read from colbeg/colend (the
-// reference loops to nnz over an n-sized buffer; that is not reproduced).
-static int read_ints(FILE* f, int64_t count, int32_t* dst) {
+// File I/O.  Files are mapped read-only and parsed in place (no stdio
+// per-integer calls), so reading is bound by the file system.
+}  // extern "C"
+
+namespace {
+
+// Read-only mapping of a whole file; len 0 maps nothing.
+struct Mapped {
+    const char* data = nullptr;
+    size_t len = 0;
+    int fd = -1;
+    ~Mapped() { close_map(); }
+    int open_map(const char* path) {
+        fd = open(path, O_RDONLY);
+        if (fd < 0) return GSOFA_ERR_IO;
+        struct stat st;
+        if (fstat(fd, &st) != 0 || st.st_size < 0) return GSOFA_ERR_IO;
+        len = (size_t)st.st_size;
+        if (len == 0) return GSOFA_OK;
+        void* p = mmap(nullptr, len, PROT_READ, MAP_PRIVATE, fd, 0);
+        if (p == MAP_FAILED) {
+            len = 0;
+            return GSOFA_ERR_IO;
+        }
+        data = (const char*)p;
+        (void)madvise(p, len, MADV_SEQUENTIAL);
+        return GSOFA_OK;
+    }
+    void close_map() {
+        if (data) munmap((void*)data, len);
+        if (fd >= 0) close(fd);
+        data = nullptr;
+        len = 0;
+        fd = -1;
+    }
+};
+
+// Whitespace-separated decimal integers, parsed from a mapped buffer.
+struct IntCursor {
+    const char* p;
+    const char* e;
+    static bool space(char c) { return c == ' ' || c == '\n' || c == '\t' || c == '\r' || c == '\f' || c == '\v'; }
+    // next integer into v; false at end of input or on a malformed token
+    bool next(long long* v) {
+        while (p < e && space(*p)) p++;
+        if (p >= e) return false;
+        bool neg = false;
+        if (*p == '-' || *p == '+') neg = *p++ == '-';
+        if (p >= e || *p < '0' || *p > '9') return false;
+        unsigned long long x = 0;
+        int digits = 0;
+        while (p < e && *p >= '0' && *p <= '9') {
+            x = x * 10 + (unsigned long long)(*p++ - '0');
+            if (++digits > 18) return false;
+        }
+        if (p < e && !space(*p)) return false;
+        *v = neg ? -(long long)x : (long long)x;
+        return true;
+    }
+};
+
+static int parse_ints(IntCursor& c, int64_t count, int32_t* dst) {
     for (int64_t i = 0; i < count; i++) {
         long long v;
-        if (fscanf(f, "%lld", &v) != 1) return GSOFA_ERR_IO;
-        if (v < INT32_MIN || v > INT32_MAX) return GSOFA_ERR_IO;
+        if (!c.next(&v) || v < INT32_MIN || v > INT32_MAX) return GSOFA_ERR_IO;
         dst[i] = (int32_t)v;
     }
     return GSOFA_OK;
 }
 
+// Binary triple: a 64-byte little-endian header, then colbeg, colend and csr
+// as int32 arrays, each starting on a 64-byte boundary.
+constexpr char kBinMagic[8] = {'G', 'S', 'O', 'F', 'A', 'C', 'S', '1'};
+struct BinHeader {
+    char magic[8];
+    uint32_t version;     // 1
+    uint32_t header_bytes;  // 64
+    int64_t n;
+    int64_t nnz;
+    uint64_t off_colbeg, off_colend, off_csr;
+    uint64_t reserved;
+};
+static_assert(sizeof(BinHeader) == 64, "binary header");
+static uint64_t align64(uint64_t x) { return (x + 63) & ~63ull; }
+static void bin_layout(int64_t n, int64_t nnz, BinHeader* h) {
+    memset(h, 0, sizeof(*h));
+    memcpy(h->magic, kBinMagic, 8);
+    h->version = 1;
+    h->header_bytes = 64;
+    h->n = n;
+    h->nnz = nnz;
+    h->off_colbeg = 64;
+    h->off_colend = align64(h->off_colbeg + 4 * (uint64_t)n);
+    h->off_csr = align64(h->off_colend + 4 * (uint64_t)n);
+}
+static uint64_t bin_size(const BinHeader& h) { return h.off_csr + 4 * (uint64_t)h.nnz; }
+
+// Validated header of a mapped binary file, or an error.
+static int bin_check(const Mapped& m, BinHeader* h) {
+    if (m.len < sizeof(BinHeader)) return GSOFA_ERR_IO;
+    memcpy(h, m.data, sizeof(BinHeader));
+    if (memcmp(h->magic, kBinMagic, 8) != 0 || h->version != 1 || h->header_bytes != 64) return GSOFA_ERR_IO;
+    if (h->n < 0 || h->nnz < 0 || h->n >= INT32_MAX || h->nnz >= INT32_MAX) return GSOFA_ERR_IO;
+    BinHeader want;
+    bin_layout(h->n, h->nnz, &want);
+    if (h->off_colbeg != want.off_colbeg || h->off_colend != want.off_colend || h->off_csr != want.off_csr)
+        return GSOFA_ERR_IO;
+    if ((uint64_t)m.len < bin_size(want)) return GSOFA_ERR_IO;
+    return GSOFA_OK;
+}
+
+}  // namespace
+
+struct gsofa_mapped {
+    Mapped m;
+};
+
+extern "C" {
+
+// Reference ASCII format (with_space_optimization/symbfact.cuh:1591-1640):
+// csr file = nnz, n, then nnz indices; colbeg/colend = n integers each,
+// whitespace separated.  Only n values are read from colbeg/colend (the
+// reference loops to nnz over an n-sized buffer; that is not reproduced).
 int gsofa_read_ascii(const char* colbeg_path, const char* colend_path, const char* csr_path,
                      gsofa_csr* out) {
     if (!out || !colbeg_path || !colend_path || !csr_path) return GSOFA_ERR_INVALID;
     memset(out, 0, sizeof(*out));
-    FILE* f = fopen(csr_path, "r");
-    if (!f) return GSOFA_ERR_IO;
+    Mapped m;
+    if (m.open_map(csr_path)) return GSOFA_ERR_IO;
+    IntCursor c{m.data, m.data + m.len};
     long long nnz = -1, n = -1;
-    if (fscanf(f, "%lld", &nnz) != 1 || fscanf(f, "%lld", &n) != 1 || nnz < 0 || n < 0 ||
-        nnz >= INT32_MAX || n >= INT32_MAX) {
-        fclose(f);
+    if (!c.next(&nnz) || !c.next(&n) || nnz < 0 || n < 0 || nnz >= INT32_MAX || n >= INT32_MAX)
         return GSOFA_ERR_IO;
-    }
     int rc = alloc_csr(out, (int32_t)n, nnz);
-    if (rc) {
-        fclose(f);
-        return rc;
-    }
-    rc = read_ints(f, nnz, out->csr);
-    fclose(f);
-    if (!rc) {
-        f = fopen(colbeg_path, "r");
-        rc = f ? read_ints(f, n, out->colbeg) : GSOFA_ERR_IO;
-        if (f) fclose(f);
-    }
-    if (!rc) {
-        f = fopen(colend_path, "r");
-        rc = f ? read_ints(f, n, out->colend) : GSOFA_ERR_IO;
-        if (f) fclose(f);
+    if (rc) return rc;
+    rc = parse_ints(c, nnz, out->csr);
+    const char* paths[2] = {colbeg_path, colend_path};
+    int32_t* arrs[2] = {out->colbeg, out->colend};
+    for (int a = 0; a < 2 && !rc; a++) {
+        Mapped f;
+        rc = f.open_map(paths[a]);
+        if (!rc) {
+            IntCursor ci{f.data, f.data + f.len};
+            rc = parse_ints(ci, n, arrs[a]);
+        }
     }
     if (rc) gsofa_csr_free(out);
     return rc;
@@ -311,5 +421,74 @@ int gsofa_write_ascii(const gsofa_matrix* A, const char* colbeg_path, const char
     }
     return GSOFA_OK;
 }
+
+int gsofa_write_binary(const gsofa_matrix* A, const char* path) {
+    if (!A || !path || A->n < 0 || A->nnz < 0 || (A->n > 0 && (!A->colbeg || !A->colend)) ||
+        (A->nnz > 0 && !A->csr))
+        return GSOFA_ERR_INVALID;
+    BinHeader h;
+    bin_layout(A->n, A->nnz, &h);
+    std::string tmp = std::string(path) + ".tmp";
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return GSOFA_ERR_IO;
+    static const char zeros[64] = {0};
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+    const int32_t* arrs[3] = {A->colbeg, A->colend, A->csr};
+    const uint64_t offs[3] = {h.off_colbeg, h.off_colend, h.off_csr};
+    const uint64_t lens[3] = {(uint64_t)A->n, (uint64_t)A->n, (uint64_t)A->nnz};
+    uint64_t at = sizeof(h);
+    for (int a = 0; a < 3 && ok; a++) {
+        if (offs[a] > at) ok = fwrite(zeros, 1, offs[a] - at, f) == offs[a] - at;
+        if (ok && lens[a]) ok = fwrite(arrs[a], 4, lens[a], f) == lens[a];
+        at = offs[a] + 4 * lens[a];
+    }
+    ok = (fclose(f) == 0) && ok;
+    if (!ok || rename(tmp.c_str(), path) != 0) {
+        remove(tmp.c_str());
+        return GSOFA_ERR_IO;
+    }
+    return GSOFA_OK;
+}
+
+int gsofa_read_binary(const char* path, gsofa_csr* out) {
+    if (!out || !path) return GSOFA_ERR_INVALID;
+    memset(out, 0, sizeof(*out));
+    Mapped m;
+    if (m.open_map(path)) return GSOFA_ERR_IO;
+    BinHeader h;
+    int rc = bin_check(m, &h);
+    if (rc) return rc;
+    rc = alloc_csr(out, (int32_t)h.n, h.nnz);
+    if (rc) return rc;
+    if (h.n) {
+        memcpy(out->colbeg, m.data + h.off_colbeg, 4 * (size_t)h.n);
+        memcpy(out->colend, m.data + h.off_colend, 4 * (size_t)h.n);
+    }
+    if (h.nnz) memcpy(out->csr, m.data + h.off_csr, 4 * (size_t)h.nnz);
+    return GSOFA_OK;
+}
+
+int gsofa_map_binary(const char* path, gsofa_mapped** handle, gsofa_matrix* view) {
+    if (!path || !handle || !view) return GSOFA_ERR_INVALID;
+    *handle = nullptr;
+    memset(view, 0, sizeof(*view));
+    gsofa_mapped* g = new (std::nothrow) gsofa_mapped();
+    if (!g) return GSOFA_ERR_ALLOC;
+    BinHeader h;
+    int rc = g->m.open_map(path) ? GSOFA_ERR_IO : bin_check(g->m, &h);
+    if (rc) {
+        delete g;
+        return rc;
+    }
+    view->n = (int32_t)h.n;
+    view->nnz = h.nnz;
+    view->colbeg = reinterpret_cast<const int32_t*>(g->m.data + h.off_colbeg);
+    view->colend = reinterpret_cast<const int32_t*>(g->m.data + h.off_colend);
+    view->csr = reinterpret_cast<const int32_t*>(g->m.data + h.off_csr);
+    *handle = g;
+    return GSOFA_OK;
+}
+
+void gsofa_unmap_binary(gsofa_mapped* handle) { delete handle; }
 
 }  // extern "C"

@@ -157,5 +157,12 @@ struct SuperParams {
     uint32_t* flag;        // n scratch
 };
 hipError_t launch_supernodes(const SuperParams& p, hipStream_t s);
+hipError_t transpose_structure(const int64_t* rowptr, const int32_t* col, int64_t nnz, int32_t n,
+                               uint32_t* cnt, int64_t* colptr, int64_t* scan_tmp, int32_t* tmp,
+                               int32_t* rows, hipStream_t s);
+hipError_t supernode_sub_count(const int32_t* sstart, int32_t ns, const int64_t* Urp, uint32_t* cnt,
+                               hipStream_t s);
+hipError_t supernode_sub_write(const int32_t* sstart, int32_t ns, const int64_t* Urp, const int32_t* Ucol,
+                               const int64_t* xsub, int32_t* sub, hipStream_t s);
 
 }  // namespace gsofa

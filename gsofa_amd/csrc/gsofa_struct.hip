@@ -322,7 +322,87 @@ inline dim3 grid_for(int64_t work, int per = SB, int cap = 65536) {
     return dim3((uint32_t)g);
 }
 
+// Column counts of a row-major structure: one increment per entry.
+__global__ void k_col_count(const int32_t* col, int64_t nnz, uint32_t* cnt) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nnz;
+         k += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&cnt[col[k]], 1u);
+}
+
+// One wave per row: entry (r, c) takes the next free place of column c.  The
+// order within a column is arbitrary until the segment sort.
+__global__ void k_col_scatter(const int64_t* rowptr, const int32_t* col, int32_t n, const int64_t* colptr,
+                              uint32_t* cursor, int32_t* rows) {
+    const int lane = (int)(threadIdx.x % 64u);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64u);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; r < n; r += waves) {
+        const int64_t e = rowptr[r + 1];
+        for (int64_t k = rowptr[r] + lane; k < e; k += 64) {
+            const int32_t c = col[k];
+            rows[colptr[c] + atomicAdd(&cursor[c], 1u)] = (int32_t)r;
+        }
+    }
+}
+
+// Subscript count of supernode s: its first row, then that row's U columns.
+__global__ void k_sub_count(const int32_t* sstart, int32_t ns, const int64_t* Urp, uint32_t* cnt) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < ns;
+         s += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t f = sstart[s];
+        cnt[s] = 1u + (uint32_t)(Urp[f + 1] - Urp[f]);
+    }
+}
+
+// One wave per supernode: sub[xsub[s]..] = f, U columns of row f (sorted).
+__global__ void k_sub_write(const int32_t* sstart, int32_t ns, const int64_t* Urp, const int32_t* Ucol,
+                            const int64_t* xsub, int32_t* sub) {
+    const int lane = (int)(threadIdx.x % 64u);
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x / 64u);
+    for (int64_t s = (int64_t)blockIdx.x * (blockDim.x / 64u) + threadIdx.x / 64u; s < ns; s += waves) {
+        const int32_t f = sstart[s];
+        const int64_t o = xsub[s], b = Urp[f], len = Urp[f + 1] - b;
+        if (lane == 0) sub[o] = f;
+        for (int64_t k = lane; k < len; k += 64) sub[o + 1 + k] = Ucol[b + k];
+    }
+}
+
 }  // namespace
+
+// Column-major copy of a row-major structure (n x n, nnz entries): colptr
+// (n+1) and the row indices of each column in increasing order.  cnt (n) and
+// tmp (nnz) are scratch; scan_tmp holds scan_tmp_words(n) words.
+hipError_t transpose_structure(const int64_t* rowptr, const int32_t* col, int64_t nnz, int32_t n,
+                               uint32_t* cnt, int64_t* colptr, int64_t* scan_tmp, int32_t* tmp,
+                               int32_t* rows, hipStream_t s) {
+    if (n <= 0) return hipMemsetAsync(colptr, 0, sizeof(int64_t), s);
+    hipError_t e = launch_fill(cnt, (uint64_t)n, 0u, s);
+    if (e != hipSuccess) return e;
+    if (nnz > 0) hipLaunchKernelGGL(k_col_count, grid_for(nnz), dim3(SB), 0, s, col, nnz, cnt);
+    e = exclusive_scan_u32_to_i64(cnt, colptr, n, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    if (nnz == 0) return hipGetLastError();
+    e = launch_fill(cnt, (uint64_t)n, 0u, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_col_scatter, grid_for(n, SB / 64), dim3(SB), 0, s, rowptr, col, n, colptr, cnt, tmp);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_sort_segments(colptr, tmp, rows, n, s);
+}
+
+// Per-supernode subscripts: counts into cnt (ns), then (after the caller's
+// exclusive scan into xsub) the lists themselves.
+hipError_t supernode_sub_count(const int32_t* sstart, int32_t ns, const int64_t* Urp, uint32_t* cnt,
+                               hipStream_t s) {
+    if (ns <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sub_count, grid_for(ns), dim3(SB), 0, s, sstart, ns, Urp, cnt);
+    return hipGetLastError();
+}
+hipError_t supernode_sub_write(const int32_t* sstart, int32_t ns, const int64_t* Urp, const int32_t* Ucol,
+                               const int64_t* xsub, int32_t* sub, hipStream_t s) {
+    if (ns <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sub_write, grid_for(ns, SB / 64), dim3(SB), 0, s, sstart, ns, Urp, Ucol, xsub, sub);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill(uint32_t* p, uint64_t count, uint32_t value, hipStream_t s) {
     if (count == 0) return hipSuccess;

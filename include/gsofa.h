@@ -174,6 +174,38 @@ int32_t gsofa_plan_matrix_count(const gsofa_plan* plan);
 /* Result of matrix `index` of the last execute, in that matrix's own numbering. */
 int gsofa_plan_fetch_matrix(gsofa_plan* plan, int32_t index, gsofa_result* out);
 
+/* ---- supernodal hand-off ------------------------------------------------------
+ * No counterpart in the reference, which stops at the fill count; this is the
+ * input a supernodal numeric factorization takes next (the reference's inputs
+ * come from SuperLU_DIST, README.md "Dataset generation").  The arrays follow
+ * SuperLU_DIST's symbolic output (Glu_persist_t xsup/supno, Glu_freeable_t
+ * compressed subscripts and column-oriented L and U), for the row supernodes
+ * of gsofa_result:
+ *   xsup/supno  supernode starts (n_super+1, last = n) and per-row ids;
+ *   xsub/sub    supernode s (rows f = xsup[s] .. l = xsup[s+1]-1): f, then the
+ *               sorted U columns of row f.  Every row i of s has U_i = those
+ *               columns above i, so the list is {f..l} followed by the block
+ *               row's columns beyond l;
+ *   xlcol/lrow  L by columns: rows i > j with j in L_i, sorted per column j;
+ *   xucol/urow  U by columns: rows i < j with j in U_i, sorted per column j.
+ * Computed on the device from the last execute of a single-matrix plan that
+ * built supernodes (every share, not counts-only); otherwise
+ * GSOFA_ERR_INVALID.  Release with gsofa_supernodal_free. */
+typedef struct gsofa_supernodal {
+    int32_t n;
+    int32_t n_super;
+    int32_t* xsup;    /* n_super+1 */
+    int32_t* supno;   /* n */
+    int64_t* xsub;    /* n_super+1 */
+    int32_t* sub;
+    int64_t* xlcol;   /* n+1 */
+    int32_t* lrow;
+    int64_t* xucol;   /* n+1 */
+    int32_t* urow;
+} gsofa_supernodal;
+int gsofa_plan_fetch_supernodal(gsofa_plan* plan, gsofa_supernodal* out);
+void gsofa_supernodal_free(gsofa_supernodal* s);
+
 /* ---- request server (continuous batching) -----------------------------------
  * No counterpart in the reference (one matrix per process run).  Requests are
  * submitted one at a time from any host thread; the server copies the matrix
@@ -228,6 +260,21 @@ int gsofa_read_ascii(const char* colbeg_path, const char* colend_path, const cha
                      gsofa_csr* out);
 int gsofa_write_ascii(const gsofa_matrix* A, const char* colbeg_path, const char* colend_path,
                       const char* csr_path);
+
+/* Binary triple: a 64-byte little-endian header ("GSOFACS1", version 1,
+ * header size 64, n, nnz, byte offsets of colbeg, colend and csr), then the
+ * three int32 arrays, each on a 64-byte boundary.  No counterpart in the
+ * reference, which reads only the ASCII triple; this is its fast form.
+ * write: to path.tmp, then renamed over path.
+ * read:  a library-owned copy (free with gsofa_csr_free).
+ * map:   a read-only mapping; `view` points into it and stays valid until
+ *        gsofa_unmap_binary.  Malformed or truncated files give
+ *        GSOFA_ERR_IO. */
+typedef struct gsofa_mapped gsofa_mapped;
+int gsofa_write_binary(const gsofa_matrix* A, const char* path);
+int gsofa_read_binary(const char* path, gsofa_csr* out);
+int gsofa_map_binary(const char* path, gsofa_mapped** handle, gsofa_matrix* view);
+void gsofa_unmap_binary(gsofa_mapped* handle);
 
 /* 5-point (nz == 1) or 7-point Laplacian pattern on an nx*ny*nz grid.  nd_leaf
  * > 0 orders it by geometric nested dissection with that leaf size (0 =
