@@ -105,13 +105,14 @@ struct Shape {
     static_assert(KR <= SL && KV <= SL, "one entry / vertex per lane of a group");
     static_assert(T <= H / 2, "hash load factor");
     static_assert(T % (GPW * KR) == 0, "relax batches tile the table");
-    static_assert(T * SL * 4 + H * 8 + T * 4 + NG * SL * 4 + 1024 <= 160 * 1024, "LDS of one CU");
+    static_assert((T + 1) * SL * 4 + H * 8 + T * 4 + NG * SL * 4 + 1024 <= 160 * 1024, "LDS of one CU");
 };
 
 template <int SL>
 struct BlkLds {
     using Z = Shape<SL>;
-    uint32_t cell[Z::T][SL];             // min cost per (round target, slot); INF = none
+    uint32_t cell[Z::T + 1][SL];         // min cost per (round target, slot); INF = none.  Row T
+                                         // is a sink for the padding of edge quads, never read
     unsigned long long hkey[Z::H];       // ((m + 1) << 32) | table index; 0 = empty
     uint32_t ent_m[Z::T];                // table index -> target (window-local) or HOLE
     uint4 eidx[Z::NG][SL / 4];           // table index of each lane's edge (SL per group)
@@ -213,6 +214,17 @@ __device__ __forceinline__ uint32_t from_lane(uint32_t x, uint32_t u) {
         if (u == (uint32_t)U) return bcast_grp<SL, U>(x);
         return from_lane<SL, U + 1>(x, u);
     }
+}
+
+// Edges 0 .. top-1 cover every group's placed edges: the OR of all groups'
+// ballot bits, as a wave-uniform count.
+template <int SL>
+__device__ __forceinline__ uint32_t wave_top(uint64_t m) {
+    if constexpr (SL <= 16) m |= m >> 32;
+    if constexpr (SL <= 8) m |= m >> 16;
+    m |= m >> SL;
+    const uint32_t g = (uint32_t)m & Shape<SL>::GM;
+    return g ? 32u - (uint32_t)__builtin_clz(g) : 0u;
 }
 
 // The calling lane group's bits of a wave ballot (bit k: lane k of the group).
@@ -318,36 +330,22 @@ __device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>
                 const bool mine = (pu >> j) & 1u;
                 uint32_t idx = NONE;
                 if (mine) idx = tab_find<SL>(S, hv.civ[u]);
-                reinterpret_cast<uint32_t*>(S.eidx[h])[j] = idx;
-                const uint32_t okm = grp_ballot<SL>(mine && idx != FULL, grp);
+                const bool placed = mine && idx != FULL;
+                // unplaced edges (past the row end, or the table is full) fold
+                // into the sink row, so every group folds whole quads
+                reinterpret_cast<uint32_t*>(S.eidx[h])[j] = placed ? idx : (uint32_t)Z::T;
+                const uint64_t wm = __ballot(placed);
+                const uint32_t okm = (uint32_t)(wm >> (SL * grp)) & Z::GM;
+                const uint32_t nq = (wave_top<SL>(wm) + 3u) >> 2;  // wave-uniform
                 wave_sync();
-                if (hv.cost[u] != INF && okm) {
+                if (hv.cost[u] != INF) {
                     const uint32_t cu = hv.cost[u];
-                    if ((okm & (okm + 1u)) == 0u) {
-                        // usual case, edges 0 .. ne-1 all placed: four indices
-                        // per LDS read, no per-edge tests
-                        const uint32_t ne = (uint32_t)__popc(okm);
-                        uint32_t k4 = 0;
-                        for (; k4 + 4u <= ne; k4 += 4u) {
-                            const uint4 id = S.eidx[h][k4 >> 2];
-                            atomicMin(&S.cell[id.x][j], cu);
-                            atomicMin(&S.cell[id.y][j], cu);
-                            atomicMin(&S.cell[id.z][j], cu);
-                            atomicMin(&S.cell[id.w][j], cu);
-                        }
-                        const uint32_t* ids = reinterpret_cast<const uint32_t*>(S.eidx[h]);
-                        for (; k4 < ne; k4++) atomicMin(&S.cell[ids[k4]][j], cu);
-                    } else {
-                        // the table filled up part-way: skip the edges not placed
-                        const uint32_t top = 32u - (uint32_t)__builtin_clz(okm);
-                        for (uint32_t k4 = 0; k4 < top; k4 += 4) {
-                            const uint4 id = S.eidx[h][k4 >> 2];
-                            const uint32_t bits = okm >> k4;
-                            if (bits & 1u) atomicMin(&S.cell[id.x][j], cu);
-                            if (bits & 2u) atomicMin(&S.cell[id.y][j], cu);
-                            if (bits & 4u) atomicMin(&S.cell[id.z][j], cu);
-                            if (bits & 8u) atomicMin(&S.cell[id.w][j], cu);
-                        }
+                    for (uint32_t q = 0; q < nq; q++) {
+                        const uint4 id = S.eidx[h][q];
+                        atomicMin(&S.cell[id.x][j], cu);
+                        atomicMin(&S.cell[id.y][j], cu);
+                        atomicMin(&S.cell[id.z][j], cu);
+                        atomicMin(&S.cell[id.w][j], cu);
                     }
                 }
                 wave_sync();
@@ -495,7 +493,7 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
     unsigned long long levels = 0;
-    for (uint32_t i = tid; i < (uint32_t)(Z::T * SL); i += BB) (&S.cell[0][0])[i] = INF;
+    for (uint32_t i = tid; i < (uint32_t)((Z::T + 1) * SL); i += BB) (&S.cell[0][0])[i] = INF;
     for (uint32_t i = tid; i < (uint32_t)Z::H; i += BB) S.hkey[i] = 0ull;
     if (tid < 4) S.acc[tid] = 0;
     if (tid == 0) {
