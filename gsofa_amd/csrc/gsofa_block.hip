@@ -428,14 +428,15 @@ __device__ void relax_phase(const Blk& B, BlkLds<SL>& S, uint32_t nbuf, uint32_t
             if (c != INF) {
                 const uint32_t l = lm[u];
                 const bool up = mu > my_src;  // above the source (symbfact.cuh:992-1009)
-                const bool first = l >= B.stale;
-                // U entry: first touch; below: the relaxation of symbfact.cuh:1018-1063
-                if (up ? first : (l > c)) {
+                // U entry: first touch (the stale label, l >= stale); below: the
+                // relaxation of symbfact.cuh:1018-1063 (l > c).  One compare:
+                // stale >= W >= 1.
+                if (l > (up ? B.stale - 1u : c)) {
                     at(B.lab0, (mu * SL + j) << 2) = c;
                     const uint32_t mo = mu + off;
                     fill = up || (l >= mo && c < mo);  // below: the label crosses below m
                     push = !up;  // the source itself never decreases (its label is off)
-                    const bool fr = !up && first;  // first reach: |R_<(s)| and TE_alg
+                    const bool fr = push && l >= B.stale;  // first reach: |R_<(s)| and TE_alg
                     t.reach += fr ? 1u : 0u;
                     t.te += fr ? du : 0u;
                 }
