@@ -321,7 +321,10 @@ __device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>
             if (fresh) hv.held = true;
         }
         if (!__ballot(hv.held)) return;  // nothing left for any group
-        // -- fold the held vertices' edges, vertex by vertex
+        // -- fold the held vertices' edges, vertex by vertex.  Bit k of each
+        // group's share of longm: vertex k's row has more edge groups after
+        // the current one (rare), so only those read back pos and end.
+        const uint64_t longm = __ballot(hv.held && j < (uint32_t)KV && hv.pos + SL < hv.end);
         bool full = false;
 #pragma unroll
         for (int u = 0; u < KV; u++) {
@@ -355,13 +358,15 @@ __device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>
                     break;
                 }
                 // next group of a long row (rare): load its targets now
-                int32_t pos = (int32_t)from_lane<SL>((uint32_t)hv.pos, u);
-                const int32_t end = (int32_t)from_lane<SL>((uint32_t)hv.end, u);
-                if (pos + SL < end) {
-                    pos += SL;
-                    pu = group_mask<SL>(end - pos);
-                    if (j == (uint32_t)u) hv.pos = pos;
-                    hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
+                if ((longm >> (SL * grp + u)) & 1u) {  // group-uniform
+                    int32_t pos = (int32_t)from_lane<SL>((uint32_t)hv.pos, u);
+                    const int32_t end = (int32_t)from_lane<SL>((uint32_t)hv.end, u);
+                    if (pos + SL < end) {
+                        pos += SL;
+                        pu = group_mask<SL>(end - pos);
+                        if (j == (uint32_t)u) hv.pos = pos;
+                        hv.civ[u] = ((pu >> j) & 1u) ? at(p.ci, (uint32_t)(pos + (int32_t)j) << 2) - B.wb : 0u;
+                    }
                 }
             }
             if (hv.held && j == (uint32_t)u) hv.pend = pu;
