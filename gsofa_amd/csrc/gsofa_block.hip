@@ -343,8 +343,9 @@ __device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>
                 wave_sync();
                 if (hv.cost[u] != INF) {
                     const uint32_t cu = hv.cost[u];
+                    const uint4* ep = S.eidx[h];
                     for (uint32_t q = 0; q < nq; q++) {
-                        const uint4 id = S.eidx[h][q];
+                        const uint4 id = ep[q];
                         atomicMin(&S.cell[id.x][j], cu);
                         atomicMin(&S.cell[id.y][j], cu);
                         atomicMin(&S.cell[id.z][j], cu);
@@ -399,8 +400,9 @@ __device__ void relax_phase(const Blk& B, BlkLds<SL>& S, uint32_t nbuf, uint32_t
         const uint32_t eb = base + grp * KR;
         // lane k: entry eb + k
         const uint32_t ek = eb + j;
-        uint32_t mk = HOLE;
-        if (j < (uint32_t)KR && ek < ntab) mk = S.ent_m[ek];
+        // lanes past KR or past the table read a valid cell and drop it
+        const uint32_t mr = S.ent_m[ek < (uint32_t)Z::T ? ek : (uint32_t)Z::T - 1u];
+        const uint32_t mk = (j < (uint32_t)KR && ek < ntab) ? mr : HOLE;
         const bool okk = mk != HOLE;
         int32_t d0 = 0, d1 = 0;
         uint32_t nm = 0;
@@ -411,13 +413,13 @@ __device__ void relax_phase(const Blk& B, BlkLds<SL>& S, uint32_t nbuf, uint32_t
             nm = at(maskn, mk << 2);
         }
         const uint32_t dd = (uint32_t)(d1 - d0);
-        // lane j: slot j's label of every entry
+        // lane j: slot j's label of every entry (a hole reads vertex 0's
+        // line: its cells are INF, so the value is never used)
         uint32_t lm[KR];
 #pragma unroll
         for (int u = 0; u < KR; u++) {
             const uint32_t mu = from_lane<SL>(mk, u);
-            lm[u] = 0u;
-            if (mu != HOLE) lm[u] = at(B.lab0, (mu * SL + j) << 2);
+            lm[u] = at(B.lab0, ((mu == HOLE ? 0u : mu) * SL + j) << 2);
         }
         uint32_t mybits = 0;    // lane k: slots that push entry k's target
         uint32_t fillbits = 0;  // lane j: bit u = slot j gains entry u's target
