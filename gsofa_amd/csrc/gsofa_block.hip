@@ -122,8 +122,8 @@ struct BlkLds {
     uint32_t src[SL];
     uint32_t qcnt[2];
     uint32_t cursor;
-    uint32_t tcnt;
-    uint32_t held;
+    uint32_t tcnt[2];                    // table entries in use, by round parity
+    uint32_t held[2];                    // some group holds unfolded edges, by round parity
     uint32_t chunk;
 };
 
@@ -135,6 +135,7 @@ struct Blk {
     uint32_t W;
     uint32_t wb;       // window base (global id of local vertex 0)
     uint32_t off, stale;
+    uint32_t rb;       // round parity: which tcnt / held belong to this round
 };
 
 // Element at a 32-bit byte offset from a workgroup-uniform base: the offset
@@ -160,7 +161,7 @@ struct TallyB {
 // table has no room left.  Linear probing over H slots; an index allocated for
 // an insert that loses its race to an equal key becomes a HOLE entry.
 template <int SL>
-__device__ __forceinline__ uint32_t tab_find(BlkLds<SL>& S, uint32_t m) {
+__device__ __forceinline__ uint32_t tab_find(BlkLds<SL>& S, uint32_t* tcnt, uint32_t m) {
     using Z = Shape<SL>;
     const unsigned long long key = (unsigned long long)(m + 1u) << 32;
     uint32_t s = (m * 2654435761u) >> (32 - Z::HB);
@@ -173,7 +174,7 @@ __device__ __forceinline__ uint32_t tab_find(BlkLds<SL>& S, uint32_t m) {
         unsigned long long cur = S.hkey[s];
         if (cur == 0ull) {
             if (mine == NONE) {
-                mine = atomicAdd(&S.tcnt, 1u);
+                mine = atomicAdd(tcnt, 1u);
                 if (mine >= (uint32_t)Z::T) {
                     mine = NONE;
                     break;
@@ -277,7 +278,7 @@ __device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>
     }
     for (;;) {
         // -- take a new batch where a group has none
-        const bool want = !hv.held && !hv.done && S.tcnt < (uint32_t)Z::T;
+        const bool want = !hv.held && !hv.done && S.tcnt[B.rb] < (uint32_t)Z::T;
         uint32_t i0 = 0;
         if (j == 0 && want) i0 = atomicAdd(&S.cursor, (uint32_t)KV);
         i0 = from_lane<SL>(i0, 0);
@@ -332,7 +333,7 @@ __device__ void combine_phase(const Blk& B, BlkLds<SL>& S, GrpVtx<Shape<SL>::KV>
             while (pu) {  // group-uniform
                 const bool mine = (pu >> j) & 1u;
                 uint32_t idx = NONE;
-                if (mine) idx = tab_find<SL>(S, hv.civ[u]);
+                if (mine) idx = tab_find<SL>(S, &S.tcnt[B.rb], hv.civ[u]);
                 const bool placed = mine && idx != FULL;
                 // unplaced edges (past the row end, or the table is full) fold
                 // into the sink row, so every group folds whole quads
@@ -393,7 +394,7 @@ __device__ void relax_phase(const Blk& B, BlkLds<SL>& S, uint32_t nbuf, uint32_t
     const uint32_t w = threadIdx.x / WAVE;
     uint32_t* maskn = B.mask0 + (uint64_t)nbuf * B.W;
     uint4* qn = B.q0 + (uint64_t)nbuf * B.W;
-    const uint32_t tc = S.tcnt;
+    const uint32_t tc = S.tcnt[B.rb];
     const uint32_t ntab = tc < (uint32_t)Z::T ? tc : (uint32_t)Z::T;
     const uint32_t off = B.off;
     for (uint32_t base = w * (Z::GPW * KR); base < ntab; base += BW * (Z::GPW * KR)) {
@@ -497,6 +498,7 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
     B.lab0 = p.lab + (uint64_t)blockIdx.x * SL * W;
     B.mask0 = p.mask + (uint64_t)blockIdx.x * 2 * W;
     B.q0 = p.qg + (uint64_t)blockIdx.x * 2 * W;
+    B.rb = 0;
     uint32_t off = p.bstate[2 * blockIdx.x];
     uint32_t since = p.bstate[2 * blockIdx.x + 1];
     TallyB t{0, 0, 0, 0};
@@ -506,7 +508,8 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
     if (tid < 4) S.acc[tid] = 0;
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
-        S.cursor = S.tcnt = S.held = 0;
+        S.cursor = 0;
+        S.tcnt[0] = S.tcnt[1] = S.held[0] = S.held[1] = 0;
     }
     for (;;) {
         if (tid == 0) S.chunk = atomicAdd(&p.ctrl->src_next, 1u);
@@ -589,20 +592,23 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
             hv.done = false;
             hv.pend = 0u;
             hv.pos = hv.end = 0;
-            for (;;) {  // rounds
+            for (;;) {  // rounds, two barriers each
                 combine_phase<SL>(B, S, hv, cur, size, t);
-                if (hv.held && j == 0) atomicOr(&S.held, 1u);
+                if (hv.held && j == 0) atomicOr(&S.held[B.rb], 1u);
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
-                const bool more = S.held != 0u || S.cursor < size;
+                const bool more = S.held[B.rb] != 0u || S.cursor < size;
                 relax_phase<SL>(B, S, nbuf, my_src, my_srcg, t);
-                __syncthreads();
+                // the next round's table, prepared while others still relax:
+                // relax never reads the keys, and the other parity's counter
+                // and flag were last read before the previous barrier
                 for (uint32_t i = tid; i < (uint32_t)Z::H; i += BB) S.hkey[i] = 0ull;
                 if (tid == 0) {
-                    S.tcnt = 0;
-                    S.held = 0;
+                    S.tcnt[B.rb ^ 1u] = 0;
+                    S.held[B.rb ^ 1u] = 0;
                 }
                 __syncthreads();
+                B.rb ^= 1u;
                 if (!more) break;
             }
             if (tid == 0) {
