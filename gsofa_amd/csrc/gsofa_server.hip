@@ -14,8 +14,11 @@
 // are its fixed arrays (CSR, fill stream, outputs; block_fixed_bytes) plus its
 // traversal arrays, which take whatever the fixed part leaves of the share
 // (the plan shrinks its grid to fit, block_grid).  A request is added while at
-// least one workgroup still fits; one that cannot fit an otherwise empty share
-// fails with GSOFA_ERR_BUDGET.
+// least one workgroup still fits.  One that cannot fit an otherwise empty
+// share but fits the whole budget runs alone: its worker waits until no batch
+// is in flight, no other worker starts one meanwhile, and the plan gets the
+// whole budget.  Only a request that does not fit the whole budget fails, with
+// GSOFA_ERR_BUDGET.
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -56,6 +59,9 @@ struct gsofa_server {
     std::mutex mu;
     std::condition_variable cv_work;  // waiting requests or shutdown
     std::condition_variable cv_done;  // a request became final
+    std::condition_variable cv_idle;  // no batch in flight
+    int32_t running = 0;              // batches in flight
+    bool exclusive = false;           // a request is waiting for or using the whole budget
     std::deque<Request*> waiting;
     std::unordered_map<int64_t, Request*> live;  // every request not yet taken
     int64_t next_ticket = 1;
@@ -84,9 +90,9 @@ uint32_t largest(const std::vector<Request*>& reqs) {
     return W;
 }
 
-// Traversal bytes left for the batch within one share (<= 0: does not fit).
-int64_t trav_room(const gsofa_server* S, const std::vector<Request*>& reqs) {
-    return S->share - fixed_bytes(reqs) - gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots);
+// Traversal bytes left for the batch within `budget` (< 0: does not fit).
+int64_t trav_room(int64_t budget, const std::vector<Request*>& reqs) {
+    return budget - fixed_bytes(reqs) - gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots);
 }
 
 void finish(gsofa_server* S, Request* r, int status) {
@@ -97,7 +103,7 @@ void finish(gsofa_server* S, Request* r, int status) {
     r->info.queued_ms = ms_between(r->t_submit, r->t_start);
 }
 
-void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id) {
+void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id, int64_t budget) {
     std::vector<gsofa_matrix> mats(batch.size());
     for (size_t i = 0; i < batch.size(); i++)
         mats[i] = {batch[i]->n, batch[i]->nnz, batch[i]->colbeg.data(), batch[i]->colend.data(),
@@ -108,8 +114,8 @@ void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id) 
     o.max_supernode_size = S->opt.max_supernode_size;
     o.flags = S->opt.flags;
     o.block_slots = gsofa::kBatchSlots;  // also for a batch of one: the sizing above assumes it
-    // the traversal arrays get what the fixed part leaves of the share
-    o.mem_budget_bytes = std::max<int64_t>(S->share - fixed_bytes(batch), 1);
+    // the traversal arrays get what the fixed part leaves of the budget
+    o.mem_budget_bytes = std::max<int64_t>(budget - fixed_bytes(batch), 1);
     gsofa_plan* P = nullptr;
     int rc = gsofa_batch_create(mats.data(), (int32_t)mats.size(), &o, &P);
     if (rc == GSOFA_OK) rc = gsofa_plan_execute(P);
@@ -142,32 +148,49 @@ void worker_main(gsofa_server* S) {
     if (hipSetDevice(S->opt.device) != hipSuccess) (void)hipGetLastError();
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        S->cv_work.wait(lk, [&] { return S->stop || !S->waiting.empty(); });
+        S->cv_work.wait(lk, [&] { return (S->stop || !S->waiting.empty()) && !S->exclusive; });
         if (S->waiting.empty()) break;  // stopping and drained
         std::vector<Request*> batch;
         while (!S->waiting.empty() && (int32_t)batch.size() < S->opt.max_batch) {
             batch.push_back(S->waiting.front());
-            if (trav_room(S, batch) < 0) {
+            if (trav_room(S->share, batch) < 0) {
                 batch.pop_back();
                 break;
             }
             S->waiting.pop_front();
         }
+        int64_t budget = S->share;
+        bool alone = false;
         if (batch.empty()) {  // the oldest request cannot fit a share on its own
             Request* r = S->waiting.front();
             S->waiting.pop_front();
-            r->t_start = Clock::now();
-            finish(S, r, GSOFA_ERR_BUDGET);
-            S->cv_done.notify_all();
-            continue;
+            if (trav_room(S->opt.mem_budget_bytes, {r}) < 0) {
+                r->t_start = Clock::now();
+                finish(S, r, GSOFA_ERR_BUDGET);
+                S->cv_done.notify_all();
+                continue;
+            }
+            // it fits the whole budget: run it alone once the device drains
+            S->exclusive = true;
+            S->cv_idle.wait(lk, [&] { return S->running == 0; });
+            batch.push_back(r);
+            budget = S->opt.mem_budget_bytes;
+            alone = true;
         }
         const int32_t id = S->next_batch++;
         const Clock::time_point t0 = Clock::now();
         for (Request* r : batch) r->t_start = t0;
-        if (!S->waiting.empty()) S->cv_work.notify_one();  // another worker may take the rest
+        S->running++;
+        if (!alone && !S->waiting.empty()) S->cv_work.notify_one();  // another worker may take the rest
         lk.unlock();
-        run_batch(S, batch, id);
+        run_batch(S, batch, id, budget);
         lk.lock();
+        S->running--;
+        if (alone) {
+            S->exclusive = false;
+            S->cv_work.notify_all();
+        }
+        if (S->running == 0) S->cv_idle.notify_all();
     }
 }
 

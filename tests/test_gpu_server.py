@@ -110,6 +110,26 @@ def test_server_budget_rejects_oversized():
     _same(r1, r3)
 
 
+def test_server_runs_oversized_alone():
+    """A request that does not fit a worker's share but fits the whole budget
+    (a 200K-row band: about 1.9 GB, mostly its reserved fill stream, against
+    1.5 GiB shares of 3 GiB) waits for the device to drain and runs alone;
+    every result equals its stand-alone run."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    small = cf.batch_matrix(20, 1)
+    big = g.gen_band(200000, 256, 8.0, 3)
+    _say("case: server budget 3 GiB in 2 shares, one request above a share")
+    with g.Server(max_inflight=2, max_batch=4, mem_gb=3.0) as srv:
+        ts = [srv.submit(small), srv.submit(big), srv.submit(small), srv.submit(small)]
+        out = [srv.wait(t) for t in ts]
+    assert out[1][1]["batch_size"] == 1
+    _same(out[1][0], g.symbfact(big))
+    alone = g.symbfact(small)
+    for k in (0, 2, 3):
+        _same(out[k][0], alone)
+
+
 def test_server_poll_and_unknown_ticket():
     import gsofa_amd as g
     import gsofa_amd.configs as cf
