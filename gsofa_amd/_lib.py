@@ -11,6 +11,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgsofa.so")
 
 GSOFA_FLAG_COUNTS_ONLY = 0x1
+GSOFA_FLAG_SPILL = 0x2
 MODE_AUTO, MODE_BLOCK, MODE_GRID, MODE_HYBRID = 0, 1, 2, 3
 
 # Every entry point include/gsofa.h declares (kept in sync by a CPU test).

@@ -19,6 +19,13 @@
 
 using namespace gsofa;
 
+// Rows of one share moved to host memory (GSOFA_FLAG_SPILL): the share's
+// row pointers over all n rows (other shares' rows are empty) and columns.
+struct SpillShare {
+    std::vector<int64_t> Lrp, Urp;
+    std::vector<int32_t> Lcol, Ucol;
+};
+
 struct gsofa_plan {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -100,6 +107,12 @@ struct gsofa_plan {
     bool executed = false;
     bool structure = false;
     bool supernodes = false;
+    bool spilled = false;               // the last execute moved its rows to host memory
+    std::vector<SpillShare> sp;         // per share, when spilled
+    std::vector<int64_t> sp_rowfill;    // n, when spilled
+    uint32_t* d_sp_lcnt = nullptr;      // n: L / U row lengths and pass-through flags
+    uint32_t* d_sp_ucnt = nullptr;      //    kept across spilled shares
+    uint32_t* d_sp_pass = nullptr;
     Ctrl stats{};
     double ms = 0, traverse_ms = 0, share_min = 0, share_max = 0;
     gsofa_result host{};      // host copy of the last execute (per-matrix fetches)
@@ -406,6 +419,9 @@ void release_device(gsofa_plan* P) {
     dfree(P->d_nsuper);
     dfree(P->d_sn64);
     dfree(P->d_snflag);
+    dfree(P->d_sp_lcnt);
+    dfree(P->d_sp_ucnt);
+    dfree(P->d_sp_pass);
     dfree(P->d_mask);
     dfree(P->d_qgv);
     dfree(P->d_bstate);
@@ -600,7 +616,8 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     }
     if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
     if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu ||
-        (P->opt.block_slots != 0 && !blk_slots_valid(P->opt.block_slots))) {
+        (P->opt.block_slots != 0 && !blk_slots_valid(P->opt.block_slots)) ||
+        ((P->opt.flags & GSOFA_FLAG_SPILL) && n_mats > 1)) {
         delete P;
         return GSOFA_ERR_INVALID;
     }
@@ -704,6 +721,8 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         e |= dalloc(&P->d_q1, (size_t)P->Q);
     }
     P->fill_cap = P->opt.output_capacity > 0 ? (uint64_t)P->opt.output_capacity : default_fill_cap(P->nnz);
+    if ((P->opt.flags & GSOFA_FLAG_SPILL) && P->opt.output_capacity <= 0)  // one share's fills at a time
+        P->fill_cap = std::max<uint64_t>(P->fill_cap / (uint64_t)std::max(1, P->opt.n_virtual_gpu), 1u << 20);
     e |= dalloc(&P->d_rp, (size_t)n + 1);
     e |= dalloc(&P->d_ci, (size_t)P->nnz);
     e |= dalloc(&P->d_fills, (size_t)P->fill_cap);
@@ -873,6 +892,226 @@ static TravParams grid_params(const gsofa_plan* P, int32_t gid) {
     return tp;
 }
 
+// One share's traversal, bracketed by events 2+2si / 3+2si.
+static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
+    const int32_t n = P->n;
+    // grid mode: stamps must increase per marker slot: move to a fresh
+    // stamp range (or clear the markers when the 32-bit range is used up)
+    if ((P->mode == GSOFA_MODE_GRID || P->hybrid) && !P->marks_clean) {
+        if ((uint64_t)(P->epoch + 2) * (uint64_t)n >= 0xFFFFFFFFull) {
+            const size_t Cm = P->hybrid ? (size_t)P->hC : (size_t)P->C;
+            HIPCHK(hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * Cm * n, P->stream));
+            P->epoch = 0;
+        } else {
+            P->epoch++;
+        }
+    }
+    P->marks_clean = false;
+    if (P->mode == GSOFA_MODE_BLOCK) {
+        BlkParams bp{};
+        bp.rp = P->d_rp;
+        bp.ci = P->d_ci;
+        bp.n = n;
+        bp.W = P->W;
+        bp.chunks = P->d_chunks + P->share_chunk_off[gid];
+        bp.n_chunks = P->share_chunk_off[gid + 1] - P->share_chunk_off[gid];
+        bp.mat_base = P->d_mat_base;
+        bp.row_te = P->d_row_te;
+        bp.row_reach = P->d_row_reach;
+        bp.nblocks = P->nblocks;
+        bp.lab = P->d_lab;
+        bp.mask = P->d_mask;
+        bp.qg = P->d_qgv;
+        bp.fills = P->d_fills;
+        bp.fill_cap = P->fill_cap;
+        bp.ctrl = P->d_ctrl;
+        bp.bstate = P->d_bstate;
+        HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
+        HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+        if (P->hybrid && P->h_g_hi > P->h_g_lo) {  // the top groups, whole device each
+            TravParams tp = grid_params(P, gid);
+            tp.chunk = P->hC;
+            tp.first_group = P->h_g_lo;
+            tp.n_groups = P->h_g_hi - P->h_g_lo;
+            tp.lab = P->d_hlab;
+            tp.lab_words = P->hlab_words;
+            tp.qcap = P->hQ;
+            tp.nblocks = P->h_nblocks;
+            HIPCHK(launch_traverse(tp, P->stream, P->coop));
+            advance_groups(P, gid, P->hC, (uint64_t)P->h_g_lo, (uint64_t)P->h_g_hi);
+        }
+        if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->S, P->blk_lds, P->stream));
+        HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+        return GSOFA_OK;
+    }
+    TravParams tp = grid_params(P, gid);
+    HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+    if (tp.n_groups > 0) HIPCHK(launch_traverse(tp, P->stream, P->coop));
+    HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+    advance_state(P, gid);
+    return GSOFA_OK;
+}
+
+// Sorted L/U rows (d_Lrp/d_Lcol, d_Urp/d_Ucol) of the rows of share gpu_id of
+// N (-1: every row) from the fill stream's n_fills entries.
+static int build_structure(gsofa_plan* P, uint64_t n_fills, int32_t N, int32_t gpu_id) {
+    const int32_t n = P->n;
+    FinalParams f{};
+    f.n = n;
+    f.rp = P->d_rp;
+    f.ci = P->d_ci;
+    f.fills = P->d_fills;
+    f.n_fills = n_fills;
+    f.n_gpu = N;
+    f.gpu_id = gpu_id;
+    f.L_rowptr = P->d_Lrp;
+    f.U_rowptr = P->d_Urp;
+    f.row_fill = P->d_rowfill;
+    f.cntL = P->d_cntL;
+    f.cntU = P->d_cntU;
+    f.scan_tmp = P->d_scan;
+    HIPCHK(launch_count(f, P->stream));
+    HIPCHK(exclusive_scan_u32_to_i64(P->d_cntL, P->d_Lrp, n, P->d_scan, P->stream));
+    HIPCHK(exclusive_scan_u32_to_i64(P->d_cntU, P->d_Urp, n, P->d_scan, P->stream));
+    int64_t tot[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    P->nnzL = tot[0];
+    P->nnzU = tot[1];
+    int rc = ensure_cols(P, P->nnzL, P->nnzU);
+    if (rc) return rc;
+    f.L_col = P->d_Ltmp;
+    f.U_col = P->d_Utmp;
+    HIPCHK(launch_scatter(f, P->stream));
+    HIPCHK(launch_sort_segments(P->d_Lrp, P->d_Ltmp, P->d_Lcol, n, P->stream));
+    HIPCHK(launch_sort_segments(P->d_Urp, P->d_Utmp, P->d_Ucol, n, P->stream));
+    return GSOFA_OK;
+}
+
+// Supernodes from d_Lrp/d_Lcol/d_Urp, or (pass != null) from d_Urp and the
+// per-row pass-through flags.
+static int build_supernodes(gsofa_plan* P, const uint32_t* pass) {
+    const int32_t n = P->n;
+    if (!P->d_supno) {
+        int e = 0;
+        e |= dalloc(&P->d_supno, (size_t)n);
+        e |= dalloc(&P->d_sstart, (size_t)n + 1);
+        e |= dalloc(&P->d_nsuper, 1);
+        e |= dalloc(&P->d_sn64, scan_tmp_words(n) + 2 * (size_t)n + 8);
+        e |= dalloc(&P->d_snflag, (size_t)n);
+        if (e) return GSOFA_ERR_ALLOC;
+    }
+    SuperParams sp{};
+    sp.n = n;
+    sp.L_rowptr = P->d_Lrp;
+    sp.L_col = P->d_Lcol;
+    sp.U_rowptr = P->d_Urp;
+    sp.max_size = P->opt.max_supernode_size;
+    sp.supno = P->d_supno;
+    sp.super_start = P->d_sstart;
+    sp.n_super = P->d_nsuper;
+    sp.tmp64 = P->d_sn64;
+    sp.flag = P->d_snflag;
+    sp.pass = pass;
+    HIPCHK(launch_supernodes(sp, P->stream));
+    return GSOFA_OK;
+}
+
+// GSOFA_FLAG_SPILL: each share runs to completion (its fill stream regrown
+// and the share rerun from the counters it started with, if it overflows),
+// then its sorted rows go to host memory and the fill stream starts over.
+// Per-row lengths and pass-through flags stay on the device for the
+// supernode pass; the rows themselves are merged at fetch.
+static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares, bool all_rows) {
+    const int32_t n = P->n;
+    const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+    if (!P->d_sp_lcnt) {
+        int e = 0;
+        e |= dalloc(&P->d_sp_lcnt, (size_t)n);
+        e |= dalloc(&P->d_sp_ucnt, (size_t)n);
+        e |= dalloc(&P->d_sp_pass, (size_t)n);
+        if (e) return GSOFA_ERR_ALLOC;
+    }
+    HIPCHK(launch_fill(P->d_sp_lcnt, (uint64_t)n, 0u, P->stream));
+    HIPCHK(launch_fill(P->d_sp_ucnt, (uint64_t)n, 0u, P->stream));
+    HIPCHK(launch_fill(P->d_sp_pass, (uint64_t)n, 0u, P->stream));
+    P->sp.assign((size_t)N, SpillShare{});
+    P->sp_rowfill.assign((size_t)n, -1);
+    std::vector<int64_t> rowfill((size_t)n);
+    Ctrl snap;
+    memset(&snap, 0, sizeof(snap));
+    HIPCHK(hipEventRecord(P->ev[0], P->stream));
+    int64_t nnzL = 0, nnzU = 0;
+    for (size_t si = 0; si < shares.size(); si++) {
+        const int32_t gid = shares[si];
+        for (int attempt = 0; attempt < 3; attempt++) {
+            HIPCHK(hipMemcpyAsync(P->d_ctrl, &snap, sizeof(Ctrl), hipMemcpyHostToDevice, P->stream));
+            int rc = run_share(P, gid, si);
+            if (rc) return rc;
+            HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
+            HIPCHK(hipStreamSynchronize(P->stream));
+            if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
+            if (P->stats.fill_pos <= P->fill_cap) break;
+            dfree(P->d_fills);
+            P->fill_cap = P->stats.fill_pos + P->stats.fill_pos / 16 + 1024;
+            if (dalloc(&P->d_fills, (size_t)P->fill_cap)) return GSOFA_ERR_ALLOC;
+        }
+        if (P->stats.fill_pos > P->fill_cap) return GSOFA_ERR_ALLOC;
+        int rc = build_structure(P, P->stats.fill_pos, N, gid);
+        if (rc) return rc;
+        HIPCHK(spill_rows(P->d_Lrp, P->d_Lcol, P->d_Urp, n, N, gid, P->d_sp_lcnt, P->d_sp_ucnt, P->d_sp_pass,
+                          P->stream));
+        SpillShare& h = P->sp[(size_t)gid];
+        h.Lrp.resize((size_t)n + 1);
+        h.Urp.resize((size_t)n + 1);
+        h.Lcol.resize((size_t)std::max<int64_t>(P->nnzL, 1));
+        h.Ucol.resize((size_t)std::max<int64_t>(P->nnzU, 1));
+        HIPCHK(hipMemcpyAsync(h.Lrp.data(), P->d_Lrp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipMemcpyAsync(h.Urp.data(), P->d_Urp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+                              P->stream));
+        if (P->nnzL)
+            HIPCHK(hipMemcpyAsync(h.Lcol.data(), P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL,
+                                  hipMemcpyDeviceToHost, P->stream));
+        if (P->nnzU)
+            HIPCHK(hipMemcpyAsync(h.Ucol.data(), P->d_Ucol, sizeof(int32_t) * (size_t)P->nnzU,
+                                  hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(rowfill.data(), P->d_rowfill, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        for (int32_t i = gid; i < n; i += N) P->sp_rowfill[(size_t)i] = rowfill[(size_t)i];
+        nnzL += P->nnzL;
+        nnzU += P->nnzU;
+        snap = P->stats;  // the next share continues the counters with an empty fill stream
+        snap.fill_pos = 0;
+    }
+    P->nnzL = nnzL;
+    P->nnzU = nnzU;
+    if (P->supernodes) {
+        // the whole matrix's row pointers from the kept lengths, then the pass
+        HIPCHK(exclusive_scan_u32_to_i64(P->d_sp_lcnt, P->d_Lrp, n, P->d_scan, P->stream));
+        HIPCHK(exclusive_scan_u32_to_i64(P->d_sp_ucnt, P->d_Urp, n, P->d_scan, P->stream));
+        int rc = build_supernodes(P, P->d_sp_pass);
+        if (rc) return rc;
+    }
+    (void)all_rows;
+    P->traverse_ms = 0;
+    P->share_min = 1e300;
+    P->share_max = 0;
+    for (size_t si = 0; si < shares.size(); si++) {
+        double t = ev_ms(P->ev[2 + 2 * si], P->ev[3 + 2 * si]);
+        P->traverse_ms += t;
+        P->share_min = std::min(P->share_min, t);
+        P->share_max = std::max(P->share_max, t);
+    }
+    HIPCHK(hipEventRecord(P->ev[1], P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    P->ms = ev_ms(P->ev[0], P->ev[1]);
+    P->executed = true;
+    return GSOFA_OK;
+}
+
 extern "C" {
 
 int gsofa_plan_execute(gsofa_plan* P) {
@@ -887,6 +1126,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
     const bool all_rows = P->opt.virtual_gpu_id < 0 || N == 1;
     P->structure = !(P->opt.flags & GSOFA_FLAG_COUNTS_ONLY);
     P->supernodes = P->structure && all_rows && n > 0;
+    P->spilled = false;
     P->executed = false;
     P->host_valid = false;
     memset(&P->stats, 0, sizeof(P->stats));
@@ -901,66 +1141,17 @@ int gsofa_plan_execute(gsofa_plan* P) {
         HIPCHK(hipEventCreate(&e));
         P->ev.push_back(e);
     }
+    if (P->structure && (P->opt.flags & GSOFA_FLAG_SPILL)) {
+        P->spilled = true;
+        return execute_spill(P, shares, all_rows);
+    }
 
     for (int attempt = 0; attempt < 3; attempt++) {
         HIPCHK(hipMemsetAsync(P->d_ctrl, 0, sizeof(Ctrl), P->stream));
         HIPCHK(hipEventRecord(P->ev[0], P->stream));
         for (size_t si = 0; si < shares.size(); si++) {
-            const int32_t gid = shares[si];
-            // grid mode: stamps must increase per marker slot: move to a fresh
-            // stamp range (or clear the markers when the 32-bit range is used up)
-            if ((P->mode == GSOFA_MODE_GRID || P->hybrid) && !P->marks_clean) {
-                if ((uint64_t)(P->epoch + 2) * (uint64_t)n >= 0xFFFFFFFFull) {
-                    const size_t Cm = P->hybrid ? (size_t)P->hC : (size_t)P->C;
-                    HIPCHK(hipMemsetAsync(P->d_mark, 0, sizeof(uint32_t) * Cm * n, P->stream));
-                    P->epoch = 0;
-                } else {
-                    P->epoch++;
-                }
-            }
-            P->marks_clean = false;
-            if (P->mode == GSOFA_MODE_BLOCK) {
-                BlkParams bp{};
-                bp.rp = P->d_rp;
-                bp.ci = P->d_ci;
-                bp.n = n;
-                bp.W = P->W;
-                bp.chunks = P->d_chunks + P->share_chunk_off[gid];
-                bp.n_chunks = P->share_chunk_off[gid + 1] - P->share_chunk_off[gid];
-                bp.mat_base = P->d_mat_base;
-                bp.row_te = P->d_row_te;
-                bp.row_reach = P->d_row_reach;
-                bp.nblocks = P->nblocks;
-                bp.lab = P->d_lab;
-                bp.mask = P->d_mask;
-                bp.qg = P->d_qgv;
-                bp.fills = P->d_fills;
-                bp.fill_cap = P->fill_cap;
-                bp.ctrl = P->d_ctrl;
-                bp.bstate = P->d_bstate;
-                HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
-                HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
-                if (P->hybrid && P->h_g_hi > P->h_g_lo) {  // the top groups, whole device each
-                    TravParams tp = grid_params(P, gid);
-                    tp.chunk = P->hC;
-                    tp.first_group = P->h_g_lo;
-                    tp.n_groups = P->h_g_hi - P->h_g_lo;
-                    tp.lab = P->d_hlab;
-                    tp.lab_words = P->hlab_words;
-                    tp.qcap = P->hQ;
-                    tp.nblocks = P->h_nblocks;
-                    HIPCHK(launch_traverse(tp, P->stream, P->coop));
-                    advance_groups(P, gid, P->hC, (uint64_t)P->h_g_lo, (uint64_t)P->h_g_hi);
-                }
-                if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->S, P->blk_lds, P->stream));
-                HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
-                continue;
-            }
-            TravParams tp = grid_params(P, gid);
-            HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
-            if (tp.n_groups > 0) HIPCHK(launch_traverse(tp, P->stream, P->coop));
-            HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
-            advance_state(P, gid);
+            int rc = run_share(P, shares[si], si);
+            if (rc) return rc;
         }
         HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
@@ -983,58 +1174,11 @@ int gsofa_plan_execute(gsofa_plan* P) {
     }
 
     if (P->structure) {
-        FinalParams f{};
-        f.n = n;
-        f.rp = P->d_rp;
-        f.ci = P->d_ci;
-        f.fills = P->d_fills;
-        f.n_fills = P->stats.fill_pos;
-        f.n_gpu = N;
-        f.gpu_id = all_rows ? -1 : P->opt.virtual_gpu_id;
-        f.L_rowptr = P->d_Lrp;
-        f.U_rowptr = P->d_Urp;
-        f.row_fill = P->d_rowfill;
-        f.cntL = P->d_cntL;
-        f.cntU = P->d_cntU;
-        f.scan_tmp = P->d_scan;
-        HIPCHK(launch_count(f, P->stream));
-        HIPCHK(exclusive_scan_u32_to_i64(P->d_cntL, P->d_Lrp, n, P->d_scan, P->stream));
-        HIPCHK(exclusive_scan_u32_to_i64(P->d_cntU, P->d_Urp, n, P->d_scan, P->stream));
-        int64_t tot[2] = {0, 0};
-        HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
-        P->nnzL = tot[0];
-        P->nnzU = tot[1];
-        int rc = ensure_cols(P, P->nnzL, P->nnzU);
+        int rc = build_structure(P, P->stats.fill_pos, N, all_rows ? -1 : P->opt.virtual_gpu_id);
         if (rc) return rc;
-        f.L_col = P->d_Ltmp;
-        f.U_col = P->d_Utmp;
-        HIPCHK(launch_scatter(f, P->stream));
-        HIPCHK(launch_sort_segments(P->d_Lrp, P->d_Ltmp, P->d_Lcol, n, P->stream));
-        HIPCHK(launch_sort_segments(P->d_Urp, P->d_Utmp, P->d_Ucol, n, P->stream));
         if (P->supernodes) {
-            if (!P->d_supno) {
-                int e = 0;
-                e |= dalloc(&P->d_supno, (size_t)n);
-                e |= dalloc(&P->d_sstart, (size_t)n + 1);
-                e |= dalloc(&P->d_nsuper, 1);
-                e |= dalloc(&P->d_sn64, scan_tmp_words(n) + 2 * (size_t)n + 8);
-                e |= dalloc(&P->d_snflag, (size_t)n);
-                if (e) return GSOFA_ERR_ALLOC;
-            }
-            SuperParams sp{};
-            sp.n = n;
-            sp.L_rowptr = P->d_Lrp;
-            sp.L_col = P->d_Lcol;
-            sp.U_rowptr = P->d_Urp;
-            sp.max_size = P->opt.max_supernode_size;
-            sp.supno = P->d_supno;
-            sp.super_start = P->d_sstart;
-            sp.n_super = P->d_nsuper;
-            sp.tmp64 = P->d_sn64;
-            sp.flag = P->d_snflag;
-            HIPCHK(launch_supernodes(sp, P->stream));
+            rc = build_supernodes(P, nullptr);
+            if (rc) return rc;
         }
     }
     HIPCHK(hipEventRecord(P->ev[1], P->stream));
@@ -1084,7 +1228,8 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         gsofa_result_free(r);
         return GSOFA_ERR_ALLOC;
     }
-    HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+    if (!P->spilled)
+        HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
     if (P->mode == GSOFA_MODE_BLOCK && !P->hybrid) {
         r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
         r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
@@ -1096,12 +1241,35 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         HIPCHK(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
                               P->stream));
     }
-    HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
-    HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
-    if (P->nnzL)
-        HIPCHK(hipMemcpyAsync(r->L_col, P->d_Lcol, sizeof(int32_t) * P->nnzL, hipMemcpyDeviceToHost, P->stream));
-    if (P->nnzU)
-        HIPCHK(hipMemcpyAsync(r->U_col, P->d_Ucol, sizeof(int32_t) * P->nnzU, hipMemcpyDeviceToHost, P->stream));
+    if (P->spilled) {
+        // rows in order, each from the share that built it (row i: share i mod N)
+        memcpy(r->row_fill, P->sp_rowfill.data(), sizeof(int64_t) * (size_t)n);
+        const int32_t N = std::max(1, P->opt.n_virtual_gpu);
+        int64_t lo = 0, uo = 0;
+        r->L_rowptr[0] = r->U_rowptr[0] = 0;
+        for (int32_t i = 0; i < n; i++) {
+            const SpillShare& h = P->sp[(size_t)(i % N)];
+            if (!h.Lrp.empty()) {
+                const int64_t lb = h.Lrp[(size_t)i], le = h.Lrp[(size_t)i + 1];
+                const int64_t ub = h.Urp[(size_t)i], ue = h.Urp[(size_t)i + 1];
+                memcpy(r->L_col + lo, h.Lcol.data() + lb, sizeof(int32_t) * (size_t)(le - lb));
+                memcpy(r->U_col + uo, h.Ucol.data() + ub, sizeof(int32_t) * (size_t)(ue - ub));
+                lo += le - lb;
+                uo += ue - ub;
+            }
+            r->L_rowptr[i + 1] = lo;
+            r->U_rowptr[i + 1] = uo;
+        }
+    } else {
+        HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
+        if (P->nnzL)
+            HIPCHK(hipMemcpyAsync(r->L_col, P->d_Lcol, sizeof(int32_t) * P->nnzL, hipMemcpyDeviceToHost,
+                                  P->stream));
+        if (P->nnzU)
+            HIPCHK(hipMemcpyAsync(r->U_col, P->d_Ucol, sizeof(int32_t) * P->nnzU, hipMemcpyDeviceToHost,
+                                  P->stream));
+    }
     if (r->row_te) {
         HIPCHK(hipStreamSynchronize(P->stream));
         for (int32_t i = 0; i < n; i++)
@@ -1228,7 +1396,7 @@ int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
     if (!P || !h) return GSOFA_ERR_INVALID;
     memset(h, 0, sizeof(*h));
     if (!P->executed) return GSOFA_ERR_STATE;
-    if (P->n_mats != 1 || !P->structure || (!P->supernodes && P->n > 0)) return GSOFA_ERR_INVALID;
+    if (P->n_mats != 1 || !P->structure || (!P->supernodes && P->n > 0) || P->spilled) return GSOFA_ERR_INVALID;
     const int32_t n = P->n;
     h->n = n;
     int32_t ns = 0;

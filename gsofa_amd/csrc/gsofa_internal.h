@@ -155,8 +155,11 @@ struct SuperParams {
     int32_t* n_super;      // 1
     int64_t* tmp64;        // scratch: 2n+1 + scan_tmp_words(n)
     uint32_t* flag;        // n scratch
+    const uint32_t* pass;  // optional, n: row i-1 is in L_i (then L_col is not read)
 };
 hipError_t launch_supernodes(const SuperParams& p, hipStream_t s);
+hipError_t spill_rows(const int64_t* Lrp, const int32_t* Lcol, const int64_t* Urp, int32_t n, int32_t n_gpu,
+                      int32_t gpu_id, uint32_t* lcnt, uint32_t* ucnt, uint32_t* pass, hipStream_t s);
 hipError_t transpose_structure(const int64_t* rowptr, const int32_t* col, int64_t nnz, int32_t n,
                                uint32_t* cnt, int64_t* colptr, int64_t* scan_tmp, int32_t* tmp,
                                int32_t* rows, hipStream_t s);

@@ -278,8 +278,13 @@ __global__ void k_sn_flags(SuperParams p, int64_t* key) {
          i += (int64_t)gridDim.x * blockDim.x) {
         bool join = false;
         if (i > 0) {
-            const int64_t lb = p.L_rowptr[i], le = p.L_rowptr[i + 1];
-            const bool pass = le > lb && p.L_col[le - 1] == (int32_t)(i - 1);
+            bool pass;
+            if (p.pass) {
+                pass = p.pass[i] != 0u;
+            } else {
+                const int64_t lb = p.L_rowptr[i], le = p.L_rowptr[i + 1];
+                pass = le > lb && p.L_col[le - 1] == (int32_t)(i - 1);
+            }
             const int64_t ui = p.U_rowptr[i + 1] - p.U_rowptr[i];
             const int64_t up = p.U_rowptr[i] - p.U_rowptr[i - 1];
             join = pass && (ui == up - 1);
@@ -320,6 +325,21 @@ inline dim3 grid_for(int64_t work, int per = SB, int cap = 65536) {
     if (g < 1) g = 1;
     if (g > cap) g = cap;
     return dim3((uint32_t)g);
+}
+
+// Rows of one share: their L and U lengths and whether row i-1 is in L_i (the
+// last, largest column of the sorted row), kept after the share's columns
+// leave the device.
+__global__ void k_spill_rows(const int64_t* Lrp, const int32_t* Lcol, const int64_t* Urp, int32_t n,
+                             int32_t n_gpu, int32_t gpu_id, uint32_t* lcnt, uint32_t* ucnt, uint32_t* pass) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (!row_processed((int32_t)i, n_gpu, gpu_id)) continue;
+        const int64_t lb = Lrp[i], le = Lrp[i + 1];
+        lcnt[i] = (uint32_t)(le - lb);
+        ucnt[i] = (uint32_t)(Urp[i + 1] - Urp[i]);
+        pass[i] = (i > 0 && le > lb && Lcol[le - 1] == (int32_t)(i - 1)) ? 1u : 0u;
+    }
 }
 
 // Column counts of a row-major structure: one increment per entry.
@@ -367,6 +387,13 @@ __global__ void k_sub_write(const int32_t* sstart, int32_t ns, const int64_t* Ur
 }
 
 }  // namespace
+
+hipError_t spill_rows(const int64_t* Lrp, const int32_t* Lcol, const int64_t* Urp, int32_t n, int32_t n_gpu,
+                      int32_t gpu_id, uint32_t* lcnt, uint32_t* ucnt, uint32_t* pass, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_spill_rows, grid_for(n), dim3(SB), 0, s, Lrp, Lcol, Urp, n, n_gpu, gpu_id, lcnt, ucnt, pass);
+    return hipGetLastError();
+}
 
 // Column-major copy of a row-major structure (n x n, nnz entries): colptr
 // (n+1) and the row indices of each column in increasing order.  cnt (n) and

@@ -69,6 +69,10 @@ void gsofa_csr_free(gsofa_csr* m);
  * supernode variant's extra knob (with_space_optimization/with_supernodes/
  * symbfact.cuh:1577).  The remaining fields replace compile-time #defines. */
 #define GSOFA_FLAG_COUNTS_ONLY 0x1  /* fill counts only: no L/U or supernodes  */
+#define GSOFA_FLAG_SPILL 0x2        /* single matrix: after each of the         */
+                                    /* n_virtual_gpu shares, move its L/U rows  */
+                                    /* to host memory, so device output holds   */
+                                    /* one share (supernodes stay on device)    */
 
 typedef struct gsofa_options {
     int32_t n_virtual_gpu;        /* sources split cyclically into this many shares (>=1) */

@@ -362,6 +362,17 @@ def test_block_slots_argument_check():
     assert g._options().block_slots == 0
 
 
+def test_spill_refused_for_batches():
+    """GSOFA_FLAG_SPILL is for single matrices; a batch with it is refused
+    before any device work."""
+    import gsofa_amd as g
+    A = g.CsrMatrix.from_rows([[0, 1], [0, 1, 2], [1, 2]])
+    with pytest.raises(g.GsofaError) as ei:
+        g.Plan([A, A], spill=True)
+    assert ei.value.code == -1
+    assert g._options(spill=True).flags & g.GSOFA_FLAG_SPILL
+
+
 def test_tune_signature_and_cache(tmp_path):
     """Autotuner host logic: signatures bucket sizes, the cache round-trips."""
     import gsofa_amd as g
