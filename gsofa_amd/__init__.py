@@ -343,7 +343,7 @@ class Server:
 
     submit(A) copies the matrix and returns a ticket; wait(ticket) / poll(ticket)
     return (SymbfactResult, info) once the request is final, where info holds
-    queued_ms, latency_ms, batch_ms, batch_id and batch_size.
+    queued_ms, latency_ms, batch_ms, batch_id, batch_size and spill_shares.
     """
 
     def __init__(self, device=0, mem_gb=0.0, max_inflight=2, max_batch=256, max_supernode_size=0,
@@ -374,7 +374,7 @@ class Server:
             _lib.lib().gsofa_result_free(C.byref(r))
         return res, {"queued_ms": info.queued_ms, "latency_ms": info.latency_ms,
                      "batch_ms": info.batch_ms, "batch_id": info.batch_id,
-                     "batch_size": info.batch_size}
+                     "batch_size": info.batch_size, "spill_shares": info.spill_shares}
 
     def poll(self, ticket):
         """None while the request runs, else (result, info)."""

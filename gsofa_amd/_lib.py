@@ -75,7 +75,7 @@ class ServerOptions(C.Structure):
 
 class RequestInfo(C.Structure):
     _fields_ = [("queued_ms", C.c_double), ("latency_ms", C.c_double), ("batch_ms", C.c_double),
-                ("batch_id", C.c_int32), ("batch_size", C.c_int32)]
+                ("batch_id", C.c_int32), ("batch_size", C.c_int32), ("spill_shares", C.c_int32)]
 
 
 _lib = None

@@ -515,21 +515,29 @@ int choose_slots(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* slo
     return GSOFA_OK;
 }
 
+// Workgroups of the block kernel with `slots` slots that are co-resident on
+// `device`.
+int resident_blocks(int slots, int device, uint32_t* out) {
+    int occ = 0, cus = 0;
+    if (!blk_slots_valid(slots)) return GSOFA_ERR_INVALID;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(slots), blk_block_size(), 0) !=
+            hipSuccess ||
+        occ <= 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
+    *out = (uint32_t)occ * (uint32_t)cus;
+    return GSOFA_OK;
+}
+
 // As many workgroups as can be co-resident (or as the options ask: chunk =
 // concurrent sources, grid_blocks), at most one per chunk, shrunk until the
 // per-workgroup arrays fit `avail` bytes (and opt.mem_budget_bytes if set).
 int block_grid(const gsofa_options& o, int S, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out) {
     if (o.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, o.mem_budget_bytes);
-    int occ = 0, cus = 0;
-    if (!blk_slots_valid(S)) return GSOFA_ERR_INVALID;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, blk_kernel_ptr(S), blk_block_size(), 0) !=
-            hipSuccess ||
-        occ <= 0 ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) {
-        (void)hipGetLastError();
-        return GSOFA_ERR_HIP;
-    }
-    uint32_t nb = (uint32_t)occ * (uint32_t)cus;
+    uint32_t nb = 0;
+    const int rc = resident_blocks(S, o.device, &nb);
+    if (rc) return rc;
     if (o.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)o.grid_blocks);
     if (o.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, o.chunk / S));
     nb = (uint32_t)std::min<uint64_t>(nb, std::max<uint64_t>(nchunks, 1));

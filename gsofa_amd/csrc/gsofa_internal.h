@@ -77,6 +77,7 @@ int64_t block_bytes(uint32_t nb, uint32_t W, int slots);
 int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap);
 uint64_t default_fill_cap(int64_t nnz);
 int block_grid(const gsofa_options& o, int slots, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
+int resident_blocks(int slots, int device, uint32_t* out);
 const void* blk_kernel_ptr(int slots);
 hipError_t launch_traverse_blk(const BlkParams& p, int slots, size_t lds_bytes, hipStream_t s);
 

@@ -15,9 +15,13 @@
 // traversal arrays, which take whatever the fixed part leaves of the share
 // (the plan shrinks its grid to fit, block_grid).  A request is added while at
 // least one workgroup still fits.  One that cannot fit an otherwise empty
-// share but fits the whole budget runs alone: its worker waits until no batch
-// is in flight, no other worker starts one meanwhile, and the plan gets the
-// whole budget.  Only a request that does not fit the whole budget fails, with
+// share runs alone: its worker waits until no batch is in flight, no other
+// worker starts one meanwhile, and the plan gets the whole budget.  Its output
+// goes in core or in spilled shares (GSOFA_FLAG_SPILL, powers of two up to
+// kMaxSpill): the fewest shares that leave room for 7/8 of the traversal
+// workgroups the most shares would (up to the resident count), since a
+// spilled share costs host copies while a missing workgroup costs traversal
+// throughput.  Only a request with room for no workgroup fails, with
 // GSOFA_ERR_BUDGET.
 #include <chrono>
 #include <condition_variable>
@@ -56,6 +60,7 @@ struct Request {
 struct gsofa_server {
     gsofa_server_options opt{};
     int64_t share = 0;  // device bytes per worker
+    uint32_t resident = 1;  // co-resident traversal workgroups of a batch plan
     std::mutex mu;
     std::condition_variable cv_work;  // waiting requests or shutdown
     std::condition_variable cv_done;  // a request became final
@@ -72,8 +77,12 @@ struct gsofa_server {
 
 namespace {
 
-// Bytes the batch `reqs` would need apart from its traversal arrays.
-int64_t fixed_bytes(const std::vector<Request*>& reqs) {
+constexpr int32_t kMaxSpill = 256;  // most spilled shares tried for one request
+
+// Bytes the batch `reqs` would need apart from its traversal arrays, with its
+// output in `shares` spilled shares (1: in core; the fill stream then holds
+// one share, as gsofa_plan_create sizes it).
+int64_t fixed_bytes(const std::vector<Request*>& reqs, int32_t shares = 1) {
     int64_t n = 0, nnz = 0;
     uint64_t nchunks = 0;
     for (const Request* r : reqs) {
@@ -81,7 +90,9 @@ int64_t fixed_bytes(const std::vector<Request*>& reqs) {
         nnz += r->nnz;
         nchunks += ((uint64_t)r->n + gsofa::kBatchSlots - 1) / gsofa::kBatchSlots;
     }
-    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, gsofa::default_fill_cap(nnz));
+    uint64_t fc = gsofa::default_fill_cap(nnz);
+    if (shares > 1) fc = std::max<uint64_t>(fc / (uint64_t)shares, 1u << 20);
+    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, fc);
 }
 
 uint32_t largest(const std::vector<Request*>& reqs) {
@@ -91,8 +102,8 @@ uint32_t largest(const std::vector<Request*>& reqs) {
 }
 
 // Traversal bytes left for the batch within `budget` (< 0: does not fit).
-int64_t trav_room(int64_t budget, const std::vector<Request*>& reqs) {
-    return budget - fixed_bytes(reqs) - gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots);
+int64_t trav_room(int64_t budget, const std::vector<Request*>& reqs, int32_t shares = 1) {
+    return budget - fixed_bytes(reqs, shares) - gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots);
 }
 
 void finish(gsofa_server* S, Request* r, int status) {
@@ -103,7 +114,7 @@ void finish(gsofa_server* S, Request* r, int status) {
     r->info.queued_ms = ms_between(r->t_submit, r->t_start);
 }
 
-void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id, int64_t budget) {
+void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id, int64_t budget, int32_t spill) {
     std::vector<gsofa_matrix> mats(batch.size());
     for (size_t i = 0; i < batch.size(); i++)
         mats[i] = {batch[i]->n, batch[i]->nnz, batch[i]->colbeg.data(), batch[i]->colend.data(),
@@ -114,8 +125,12 @@ void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id, 
     o.max_supernode_size = S->opt.max_supernode_size;
     o.flags = S->opt.flags;
     o.block_slots = gsofa::kBatchSlots;  // also for a batch of one: the sizing above assumes it
+    if (spill > 1) {  // a single request, alone on the device
+        o.flags |= GSOFA_FLAG_SPILL;
+        o.n_virtual_gpu = spill;
+    }
     // the traversal arrays get what the fixed part leaves of the budget
-    o.mem_budget_bytes = std::max<int64_t>(budget - fixed_bytes(batch), 1);
+    o.mem_budget_bytes = std::max<int64_t>(budget - fixed_bytes(batch, spill), 1);
     gsofa_plan* P = nullptr;
     int rc = gsofa_batch_create(mats.data(), (int32_t)mats.size(), &o, &P);
     if (rc == GSOFA_OK) rc = gsofa_plan_execute(P);
@@ -136,6 +151,7 @@ void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id, 
         r->info.batch_id = batch_id;
         r->info.batch_size = (int32_t)batch.size();
         r->info.batch_ms = batch_ms;
+        r->info.spill_shares = spill;
         std::vector<int32_t>().swap(r->colbeg);  // the input copy is no longer needed
         std::vector<int32_t>().swap(r->colend);
         std::vector<int32_t>().swap(r->csr);
@@ -160,17 +176,30 @@ void worker_main(gsofa_server* S) {
             S->waiting.pop_front();
         }
         int64_t budget = S->share;
+        int32_t spill = 1;
         bool alone = false;
         if (batch.empty()) {  // the oldest request cannot fit a share on its own
             Request* r = S->waiting.front();
             S->waiting.pop_front();
-            if (trav_room(S->opt.mem_budget_bytes, {r}) < 0) {
+            const std::vector<Request*> one{r};
+            const int32_t most = std::min<int32_t>(kMaxSpill, std::max<int32_t>(r->n, 1));
+            const int64_t per_wg = gsofa::block_bytes(1, largest(one), gsofa::kBatchSlots);
+            auto wgs = [&](int32_t sh) {  // traversal workgroups that fit with sh shares
+                const int64_t room = S->opt.mem_budget_bytes - fixed_bytes(one, sh);
+                return room < per_wg ? (int64_t)0 : std::min<int64_t>(room / per_wg, (int64_t)S->resident);
+            };
+            int32_t top = 1;
+            while (top * 2 <= most) top *= 2;
+            const int64_t best = wgs(top);
+            while (spill < top && 8 * wgs(spill) < 7 * best) spill *= 2;
+            if (best == 0) {
                 r->t_start = Clock::now();
                 finish(S, r, GSOFA_ERR_BUDGET);
                 S->cv_done.notify_all();
                 continue;
             }
-            // it fits the whole budget: run it alone once the device drains
+            // it fits the whole budget (in core or spilled): run it alone once
+            // the device drains
             S->exclusive = true;
             S->cv_idle.wait(lk, [&] { return S->running == 0; });
             batch.push_back(r);
@@ -183,7 +212,7 @@ void worker_main(gsofa_server* S) {
         S->running++;
         if (!alone && !S->waiting.empty()) S->cv_work.notify_one();  // another worker may take the rest
         lk.unlock();
-        run_batch(S, batch, id, budget);
+        run_batch(S, batch, id, budget, spill);
         lk.lock();
         S->running--;
         if (alone) {
@@ -241,8 +270,15 @@ int gsofa_server_create(const gsofa_server_options* opt, gsofa_server** out) {
         }
         o.mem_budget_bytes = (int64_t)((double)freeb * 0.80);
     }
+    uint32_t resident = 0;
+    if (hipSetDevice(o.device) != hipSuccess ||
+        gsofa::resident_blocks(gsofa::kBatchSlots, o.device, &resident) != GSOFA_OK) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
     gsofa_server* S = new gsofa_server();
     S->opt = o;
+    S->resident = std::max<uint32_t>(resident, 1);
     S->share = o.mem_budget_bytes / o.max_inflight;
     for (int32_t i = 0; i < o.max_inflight; i++) S->workers.emplace_back(worker_main, S);
     *out = S;

@@ -240,6 +240,7 @@ typedef struct gsofa_request_info {
     double batch_ms;    /* device time of its batch (traversal + structure + supernodes) */
     int32_t batch_id;   /* 1, 2, ... in admission order */
     int32_t batch_size; /* requests in its batch */
+    int32_t spill_shares; /* 1: in core; N > 1: ran alone in N spilled shares */
 } gsofa_request_info;
 
 void gsofa_server_options_default(gsofa_server_options* opt);

@@ -91,14 +91,15 @@ def test_server_concurrent_submitters():
 
 
 def test_server_budget_rejects_oversized():
-    """A request that cannot fit a worker's share on its own fails with
-    GSOFA_ERR_BUDGET; the others complete."""
+    """A request that fits the whole budget neither in core nor in the most
+    spilled shares (a 200K-row band needs about 80 MB even in 256 shares,
+    against 0.05 GiB) fails with GSOFA_ERR_BUDGET; the others complete."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     small = cf.batch_matrix(20, 1)
     big = g.gen_band(200000, 256, 8.0, 3)
-    _say("case: server budget 0.25 GiB, one oversized request")
-    with g.Server(max_inflight=1, max_batch=4, mem_gb=0.25) as srv:
+    _say("case: server budget 0.05 GiB, one oversized request")
+    with g.Server(max_inflight=1, max_batch=4, mem_gb=0.05) as srv:
         t1 = srv.submit(small)
         t2 = srv.submit(big)
         t3 = srv.submit(small)
@@ -128,6 +129,25 @@ def test_server_runs_oversized_alone():
     alone = g.symbfact(small)
     for k in (0, 2, 3):
         _same(out[k][0], alone)
+
+
+def test_server_spills_what_does_not_fit_in_core():
+    """A request whose in-core output does not fit the whole budget (a
+    100K-row band reserves about 0.95 GB for its fill stream and columns,
+    against 0.5 GiB) runs alone in spilled shares and equals its in-core
+    stand-alone run."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    small = cf.batch_matrix(20, 1)
+    big = g.gen_band(100000, 64, 8.0, 3)
+    _say("case: server budget 0.5 GiB, one request served in spilled shares")
+    with g.Server(max_inflight=1, max_batch=4, mem_gb=0.5) as srv:
+        ts = [srv.submit(small), srv.submit(big), srv.submit(small)]
+        out = [srv.wait(t) for t in ts]
+    assert out[1][1]["spill_shares"] > 1 and out[1][1]["batch_size"] == 1
+    assert out[0][1]["spill_shares"] == 1
+    _same(out[1][0], g.symbfact(big))
+    _same(out[0][0], out[2][0])
 
 
 def test_server_poll_and_unknown_ticket():
