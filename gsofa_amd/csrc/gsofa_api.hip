@@ -594,7 +594,9 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     int S = 0;
     if (choose_slots(o, 1, share_count(A->n, N, 0), &S) != GSOFA_OK) return -1;
     const uint64_t nchunks = (share_count(A->n, N, 0) + S - 1) / S;
-    const uint64_t fc = o.output_capacity > 0 ? (uint64_t)o.output_capacity : default_fill_cap(A->nnz);
+    uint64_t fc = o.output_capacity > 0 ? (uint64_t)o.output_capacity : default_fill_cap(A->nnz);
+    if ((o.flags & GSOFA_FLAG_SPILL) && o.output_capacity <= 0)  // one share's fills at a time
+        fc = std::max<uint64_t>(fc / (uint64_t)N, 1u << 20);
     uint32_t nb = 0;
     if (block_grid(o, S, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
     return block_bytes(nb, (uint32_t)A->n, S) + block_fixed_bytes(A->n, A->nnz, 1, nchunks, fc);

@@ -15,7 +15,7 @@
 static int usage(const char* argv0) {
     fprintf(stderr,
             "usage: %s colbeg colend csr N_GPU MEM_GB [max_supernode_size [N_GPU_Node]]\n"
-            "       [--chunk C] [--share G] [--counts-only] [--device D]\n",
+            "       [--chunk C] [--share G] [--counts-only] [--spill] [--device D]\n",
             argv0);
     return 1;
 }
@@ -30,6 +30,7 @@ int main(int argc, char** argv) {
         else if (!strcmp(argv[i], "--share") && i + 1 < argc) opt.virtual_gpu_id = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--device") && i + 1 < argc) opt.device = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--counts-only")) opt.flags |= GSOFA_FLAG_COUNTS_ONLY;
+        else if (!strcmp(argv[i], "--spill")) opt.flags |= GSOFA_FLAG_SPILL;
         else if (npos < 7) pos[npos++] = argv[i];
         else return usage(argv[0]);
     }
