@@ -29,13 +29,13 @@ def log(*a):
 def workload(name, n_override=0):
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    if name == "c3":
+    if name == "band":
         n = n_override or 659033
-        return cf.config3(n), "band n=%d |i-j|<=256 Poisson(8) seed 3" % n
-    if name == "c2":
-        return cf.config2(), "3D 7-point 48^3 nested dissection + 5% extra, seed 2"
-    if name == "c1":
-        return cf.config1(), "2D 5-point 64x64 natural order"
+        return cf.band(n), "band n=%d |i-j|<=256 Poisson(8) seed 3" % n
+    if name == "grid3d_nd":
+        return cf.grid3d_nd(), "3D 7-point 48^3 nested dissection + 5% extra, seed 2"
+    if name == "grid64":
+        return cf.grid64_natural(), "2D 5-point 64x64 natural order"
     if name == "bcsstk18":
         import numpy as np
         d = np.load(os.path.join(ROOT, "tests", "golden", "bcsstk18.npz"))
@@ -82,7 +82,7 @@ def copy_probe(nbytes=1 << 30, reps=3):
 
 
 def batch_bench(steps, warmup):
-    """C4: 256 ND-ordered 2D grids (side log-uniform in [16,256]).
+    """Batch: 256 ND-ordered 2D grids (side log-uniform in [16,256]).
 
     batched:    one plan over all 256 matrices, one persistent launch per step
     one_by_one: the plain path, one plan and one launch per matrix; each plan is
@@ -93,7 +93,7 @@ def batch_bench(steps, warmup):
 
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    mats = cf.config4()
+    mats = cf.grid2d_nd_batch()
     with g.Plan(mats) as p:
         for _ in range(warmup):
             p.execute()
@@ -114,7 +114,7 @@ def batch_bench(steps, warmup):
     gold_ok = None
     gpath = os.path.join(ROOT, "tests", "golden", "goldens.json")
     if os.path.exists(gpath):
-        gold = json.load(open(gpath)).get("c4")
+        gold = json.load(open(gpath)).get("grid2d_nd_batch")
         if gold:
             gold_ok = [x["fill_count"] for x in gold] == fills
     lat_ms = np.array(lat) * 1e3
@@ -131,7 +131,7 @@ def batch_bench(steps, warmup):
 
 
 def serve_bench(count, rate, inflight=2, max_batch=64):
-    """C5: a seeded Poisson stream (rate req/s) of C4-family 2D grids and 3D
+    """Serving: a seeded Poisson stream (rate req/s) of the batch family's 2D grids and 3D
     7-point ND grids (side 16..64), submitted at its arrival times to the
     request server (continuous batching, `inflight` batches on the device) and,
     as the one-by-one baseline, to a server that runs one request per launch.
@@ -165,7 +165,7 @@ def serve_bench(count, rate, inflight=2, max_batch=64):
     lat = np.array([i["latency_ms"] for _, i in served])
     lat1 = np.array([i["latency_ms"] for _, i in alone])
     te = sum(r.te_alg for r, _ in served)
-    return {"workload": "Poisson stream, %d requests at %.0f req/s, seed 5: C4-family 2D and "
+    return {"workload": "Poisson stream, %d requests at %.0f req/s, seed 5: 2D ND grids (side 16..256) and "
                         "3D 7-point ND grids (side 16..64)" % (count, rate),
             "requests": count, "kinds_3d": sum(k for _, k, _, _ in reqs),
             "max_inflight": inflight, "max_batch": max_batch,
@@ -186,7 +186,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c3")
+    ap.add_argument("--workload", default="band", choices=["band", "grid3d_nd", "grid64", "bcsstk18"])
     ap.add_argument("--n", type=int, default=0, help="override n for the band workload")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--grid-blocks", type=int, default=0)

@@ -1,56 +1,61 @@
-"""The five benchmark workloads of this path, with fixed seeds.
+"""The benchmark workloads of this path, with fixed seeds.
 
-C1  5-point 2D Laplacian pattern, 64 x 64 grid, natural row-major order (n=4,096).
-C2  7-point 3D Laplacian, 48^3 grid, geometric nested dissection (leaf 64),
-    each row gains with probability 0.05 one extra one-directional entry to a
-    grid point at Manhattan distance 2 (n=110,592).
-C3  n=659,033 banded unsymmetric: diagonal + Poisson(8) off-diagonals drawn
-    uniformly from |i-j| <= 256, duplicates removed (the original test scale).
-C4  batch of 256 2D 5-point grids, side log-uniform in [16,256], nested
-    dissection, per-row Bernoulli(0.05) extra entry at distance 2..3.
-C5  request stream: Poisson arrivals; each request is a C4-family matrix or a
-    3D 7-point nested-dissection grid with side uniform in [16,64].
+grid64_natural   5-point 2D Laplacian pattern, 64 x 64 grid, natural row-major
+                 order (n=4,096).
+grid3d_nd        7-point 3D Laplacian, 48^3 grid, geometric nested dissection
+                 (leaf 64); each row gains with probability 0.05 one extra
+                 one-directional entry to a grid point at Manhattan distance 2
+                 (n=110,592).
+band             n=659,033 banded unsymmetric: diagonal + Poisson(8)
+                 off-diagonals drawn uniformly from |i-j| <= 256, duplicates
+                 removed (the original test scale).
+grid2d_nd_batch  256 2D 5-point grids, side log-uniform in [16,256], nested
+                 dissection, per-row Bernoulli(0.05) extra entry at distance
+                 2..3.
+stream           request stream: Poisson arrivals; each request is a matrix of
+                 the grid2d_nd_batch family or a 3D 7-point nested-dissection
+                 grid with side uniform in [16,64].
 """
 import math
 
 from . import _lib, gen_band, gen_grid
 
-SEEDS = {"c1": 1, "c2": 2, "c3": 3, "c4": 4, "c5": 5}
+SEEDS = {"grid64": 1, "grid3d": 2, "band": 3, "batch": 4, "stream": 5}
 ND_LEAF = 64
 
 
-def config1():
-    return gen_grid(64, 64, 1, nd_leaf=0, seed=SEEDS["c1"])
+def grid64_natural():
+    return gen_grid(64, 64, 1, nd_leaf=0, seed=SEEDS["grid64"])
 
 
-def config2(side=48):
+def grid3d_nd(side=48):
     return gen_grid(side, side, side, nd_leaf=ND_LEAF, extra_prob=0.05, extra_dist=2,
-                    seed=SEEDS["c2"])
+                    seed=SEEDS["grid3d"])
 
 
-def config3(n=659033):
-    return gen_band(n, 256, 8.0, SEEDS["c3"])
+def band(n=659033):
+    return gen_band(n, 256, 8.0, SEEDS["band"])
 
 
-def batch_sides(count=256, seed=SEEDS["c4"], lo=16, hi=256):
+def batch_sides(count=256, seed=SEEDS["batch"], lo=16, hi=256):
     L = _lib.lib()
     return [int(L.gsofa_gen_side_loguniform(seed, b, lo, hi)) for b in range(count)]
 
 
-def batch_matrix(side, index, seed=SEEDS["c4"]):
+def batch_matrix(side, index, seed=SEEDS["batch"]):
     return gen_grid(side, side, 1, nd_leaf=ND_LEAF, extra_prob=0.05, extra_dist=3,
                     seed=seed * 1000003 + index)
 
 
-def config4(count=256, seed=SEEDS["c4"], max_side=256):
+def grid2d_nd_batch(count=256, seed=SEEDS["batch"], max_side=256):
     sides = batch_sides(count, seed, 16, max_side)
     return [batch_matrix(s, b, seed) for b, s in enumerate(sides)]
 
 
-def stream(count, rate, seed=SEEDS["c5"]):
+def stream(count, rate, seed=SEEDS["stream"]):
     """Deterministic request stream: list of (arrival_s, kind, side, matrix).
 
-    kind 0 = C4-family 2D grid (side log-uniform [16,256]), kind 1 = 3D 7-point
+    kind 0 = grid2d_nd_batch-family 2D grid (side log-uniform [16,256]), kind 1 = 3D 7-point
     ND grid (side uniform [16,64]).  Inter-arrival times are exponential with
     the given rate (requests per second).
     """

@@ -126,10 +126,10 @@ def main(argv=None):
     ap.add_argument("--out", default=None, help="also write the records to this JSON file")
     ap.add_argument("--cache", default=DEFAULT_CACHE)
     args = ap.parse_args(argv)
-    work = [("grid64 natural (C1)", cf.config1(), 1),
-            ("3D 48^3 ND (C2)", cf.config2(), 1),
-            ("band n=659,033 (C3)", cf.config3(), 1),
-            ("256 ND 2D grids (C4 batch)", cf.config4(), 1)]
+    work = [("grid64 natural", cf.grid64_natural(), 1),
+            ("3D 48^3 ND", cf.grid3d_nd(), 1),
+            ("band n=659,033", cf.band(), 1),
+            ("batch of 256 ND 2D grids", cf.grid2d_nd_batch(), 1)]
     out = []
     for name, A, shares in work:
         best, timings = tune(A, shares=shares, reps=1, cache=args.cache,

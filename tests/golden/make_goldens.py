@@ -1,6 +1,6 @@
 """Compute oracle goldens for the large workloads and write them as JSON.
 
-Run in this repository (CPU only):  python tests/golden/make_goldens.py [c2] [c3] [c4]
+Run in this repository (CPU only):  python tests/golden/make_goldens.py [grid3d_nd] [band] [grid2d_nd_batch]
 
 For each workload the CPU oracle (oracle/liboracle.so) produces the full
 structure; the fixture keeps its totals and SHA-256 digests of the sorted L/U
@@ -35,21 +35,21 @@ def run(name, A, out):
 def main(which):
     path = os.path.join(HERE, "goldens.json")
     out = json.load(open(path)) if os.path.exists(path) else {}
-    if "c2" in which:
-        run("c2", cf.config2(), out)
-    if "c4" in which:
+    if "grid3d_nd" in which:
+        run("grid3d_nd", cf.grid3d_nd(), out)
+    if "grid2d_nd_batch" in which:
         recs = []
-        for b, A in enumerate(cf.config4()):
+        for b, A in enumerate(cf.grid2d_nd_batch()):
             r = oracle.symbfact(A.colbeg, A.colend, A.csr, A.n)
             rec = oracle_record(A, r)
             recs.append(rec)
-        out["c4"] = recs
-        print("c4", len(recs), flush=True)
-    if "c3" in which:
-        run("c3", cf.config3(), out)
+        out["grid2d_nd_batch"] = recs
+        print("grid2d_nd_batch", len(recs), flush=True)
+    if "band" in which:
+        run("band", cf.band(), out)
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["c2", "c4", "c3"])
+    main(sys.argv[1:] or ["grid3d_nd", "grid2d_nd_batch", "band"])

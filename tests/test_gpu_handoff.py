@@ -75,14 +75,14 @@ def test_handoff_small_cases(bcsstk18):
             assert np.array_equal(r2.L_col, r.L_col) and np.array_equal(r2.U_col, r.U_col)
 
 
-def test_handoff_c3_properties():
+def test_handoff_band_properties():
     """Full size (n=659,033, 2.5e8 entries): counts, per-column order and an
     order-independent checksum of the (row, column) pairs equal between the
     row-major result and the column-major hand-off."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    A = cf.config3()
-    _say("case: hand-off C3 full size")
+    A = cf.band()
+    _say("case: hand-off band n=659,033 full size")
     with g.Plan(A) as p:
         p.execute()
         r = p.fetch()

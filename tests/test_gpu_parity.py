@@ -45,7 +45,7 @@ def test_bcsstk18_reference_total(bcsstk18, mode):
 def test_grid64(mode):
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    A = cf.config1()
+    A = cf.grid64_natural()
     _say("case: grid64 natural, mode", mode)
     r = g.symbfact(A, mode=mode)
     assert r.fill_count == 500094
@@ -207,12 +207,12 @@ def _golden(name):
 
 
 @pytest.mark.parametrize("mode", MODES)
-def test_c2_full_golden(mode):
+def test_grid3d_nd_full_golden(mode):
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    gold = _golden("c2")
-    A = cf.config2()
-    _say("case: C2 48^3 ND full size, mode", mode)
+    gold = _golden("grid3d_nd")
+    A = cf.grid3d_nd()
+    _say("case: 3D 48^3 ND full size, mode", mode)
     r = g.symbfact(A, mode=mode)
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
@@ -222,13 +222,13 @@ def test_c2_full_golden(mode):
     assert array_digest(r.super_start) == gold["super0"]
 
 
-def test_c3_full_golden():
+def test_band_full_golden():
     """n=659,033 banded matrix: totals, L/U digest and supernodes vs the oracle."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    gold = _golden("c3")
-    A = cf.config3()
-    _say("case: C3 band n=659,033 full size, block mode")
+    gold = _golden("band")
+    A = cf.band()
+    _say("case: band n=659,033 full size, block mode")
     r = g.symbfact(A, mode="block")
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
@@ -245,27 +245,27 @@ def _check_gold(rec, gold, tag):
         assert rec[k] == gold[k], (tag, k)
 
 
-def test_c4_one_by_one_goldens():
+def test_batch_one_by_one_goldens():
     """The 256 matrices of the batch workload, one plan each, vs oracle goldens."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    golds = _golden("c4")
-    mats = cf.config4()
+    golds = _golden("grid2d_nd_batch")
+    mats = cf.grid2d_nd_batch()
     assert len(golds) == len(mats) == 256
     for b in range(0, 256, 8):
-        _say("case: C4 matrix", b, "n", mats[b].n, "alone")
+        _say("case: batch matrix", b, "n", mats[b].n, "alone")
         _check_gold(record_of(g.symbfact(mats[b])), golds[b], b)
 
 
-def test_c4_batch_one_launch():
+def test_batch_one_launch():
     """All 256 matrices in one plan / one persistent launch: every matrix's
     result equals its golden (and so its stand-alone result)."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     from digests import array_digest
-    golds = _golden("c4")
-    mats = cf.config4()
-    _say("case: C4 batch of 256 in one plan")
+    golds = _golden("grid2d_nd_batch")
+    mats = cf.grid2d_nd_batch()
+    _say("case: batch of 256 in one plan")
     with g.Plan(mats) as p:
         assert p.matrix_count == 256
         for it in range(2):
@@ -319,7 +319,7 @@ def test_hybrid_top_sources(bcsstk18, top):
             assert_same_structure(r, o)
 
 
-def test_hybrid_random_and_c2():
+def test_hybrid_random_and_grid3d_nd():
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     rng = np.random.default_rng(91)
@@ -329,9 +329,9 @@ def test_hybrid_random_and_c2():
         top = int(rng.integers(1, n + 1))
         _say("case: random hybrid n", n, "top", top)
         assert_same_structure(g.symbfact(A, top_sources=top), oracle_of(A))
-    gold = _golden("c2")
-    _say("case: C2 48^3 ND hybrid top 8192")
-    r = g.symbfact(cf.config2(), top_sources=8192)
+    gold = _golden("grid3d_nd")
+    _say("case: 3D 48^3 ND hybrid top 8192")
+    r = g.symbfact(cf.grid3d_nd(), top_sources=8192)
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
         assert rec[k] == gold[k], k

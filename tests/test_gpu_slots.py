@@ -1,5 +1,6 @@
 """Block mode with 16 and 8 source slots per workgroup: every output
-bit-exact against the CPU oracle (small cases) or the committed goldens (C2),
+bit-exact against the CPU oracle (small cases) or the committed golden of the
+3D 48^3 ND grid,
 across the code paths the 32-slot kernel is tested on (multi-round levels,
 batches, virtual-GPU shares, the hybrid split, repeated executes)."""
 import json
@@ -115,17 +116,17 @@ def test_slots_shares_and_hybrid(bcsstk18):
 
 
 @pytest.mark.parametrize("slots", SLOTS)
-def test_slots_c2_full_golden(slots):
+def test_slots_grid3d_nd_full_golden(slots):
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     from digests import array_digest
     if not os.path.exists(GOLD):
         pytest.skip("goldens not generated")
-    gold = json.load(open(GOLD)).get("c2")
+    gold = json.load(open(GOLD)).get("grid3d_nd")
     if gold is None:
-        pytest.skip("golden c2 not generated")
-    _say("case: C2 48^3 ND full size, slots", slots)
-    r = g.symbfact(cf.config2(), block_slots=slots)
+        pytest.skip("golden grid3d_nd not generated")
+    _say("case: 3D 48^3 ND full size, slots", slots)
+    r = g.symbfact(cf.grid3d_nd(), block_slots=slots)
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
         assert rec[k] == gold[k], k
@@ -145,7 +146,7 @@ def test_auto_slots(bcsstk18):
     r = g.symbfact(A, counts_only=True)
     assert r.chunk // r.grid_blocks == 8
     assert r.fill_count == oracle_of(A).fill_count
-    mats = cf.config4()[:16]
+    mats = cf.grid2d_nd_batch()[:16]
     with g.Plan(mats, counts_only=True) as p:
         p.execute()
         r = p.fetch_matrix(0)

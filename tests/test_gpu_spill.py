@@ -1,7 +1,7 @@
 """GSOFA_FLAG_SPILL: the shares run in turn and each share's L/U rows move to
 host memory before the next, so the device holds one share's fill stream and
 rows.  Every output is bit-exact against the oracle (small cases), the
-in-core run and the committed C2 golden, including supernodes, which are
+in-core run and the committed 3D 48^3 ND golden, including supernodes, which are
 built on the device from the per-row summaries the spilled shares leave."""
 import json
 import os
@@ -77,17 +77,17 @@ def test_spill_refuses_handoff(bcsstk18):
         assert ei.value.code == -1
 
 
-def test_spill_c2_golden():
+def test_spill_grid3d_nd_golden():
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     from digests import array_digest
     if not os.path.exists(GOLD):
         pytest.skip("goldens not generated")
-    gold = json.load(open(GOLD)).get("c2")
+    gold = json.load(open(GOLD)).get("grid3d_nd")
     if gold is None:
-        pytest.skip("golden c2 not generated")
-    _say("case: C2 48^3 ND full size, spill in 4 shares")
-    r = g.symbfact(cf.config2(), n_virtual_gpu=4, spill=True)
+        pytest.skip("golden grid3d_nd not generated")
+    _say("case: 3D 48^3 ND full size, spill in 4 shares")
+    r = g.symbfact(cf.grid3d_nd(), n_virtual_gpu=4, spill=True)
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
         assert rec[k] == gold[k], k

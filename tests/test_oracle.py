@@ -69,7 +69,7 @@ def test_bcsstk18_symbolic_cholesky(bcsstk18, oracle_mod):
 
 def test_grid64_closed_forms(oracle_mod):
     import gsofa_amd.configs as cf
-    A = cf.config1()
+    A = cf.grid64_natural()
     n = A.n
     r = oracle_mod.symbfact(A.colbeg, A.colend, A.csr, n)
     assert r.fill_count == 500094
@@ -243,11 +243,11 @@ def test_share_split_is_a_partition(oracle_mod, bcsstk18):
 # ------------------------------------------------------------ generators ----
 def test_generator_fingerprints():
     import gsofa_amd.configs as cf
-    assert cf.config1().fingerprint() == "ac2231caa27d891dcf1df8dac36ec1c69f053938b131323f61a848a807a80fb1"
-    A2 = cf.config2()
+    assert cf.grid64_natural().fingerprint() == "ac2231caa27d891dcf1df8dac36ec1c69f053938b131323f61a848a807a80fb1"
+    A2 = cf.grid3d_nd()
     assert (A2.n, A2.nnz) == (110592, 765883)
     assert A2.fingerprint() == "975e7eccefe6191bfb2377b623a82c046deaac43f4a799c98eec6ea814aaaa95"
-    assert cf.config3(20000).fingerprint() == "da1b1f02c747829033ec33420d54b30111148fc3d70a228217e6c49a1866b725"
+    assert cf.band(20000).fingerprint() == "da1b1f02c747829033ec33420d54b30111148fc3d70a228217e6c49a1866b725"
     sides = cf.batch_sides()
     assert hashlib.sha256(json.dumps(sides).encode()).hexdigest() == \
         "e4f302f6262612110c71e4a0f62f58d332d90b1da4c2fa6fcff83eb1d8295976"
