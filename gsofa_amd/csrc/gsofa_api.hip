@@ -19,10 +19,11 @@
 
 using namespace gsofa;
 
-// Rows of one share moved to host memory (GSOFA_FLAG_SPILL): the share's
-// row pointers over all n rows (other shares' rows are empty) and columns.
+// Rows of one share moved to host memory (GSOFA_FLAG_SPILL): the share owns
+// rows g, g + N, g + 2N, ...; its k-th row is Lcol[Loff[k] .. Loff[k+1]) (and
+// the same for U).  Host memory is O(n + nnz(L+U)) whatever N is.
 struct SpillShare {
-    std::vector<int64_t> Lrp, Urp;
+    std::vector<int64_t> Loff, Uoff;
     std::vector<int32_t> Lcol, Ucol;
 };
 
@@ -1048,7 +1049,7 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares, bool
     HIPCHK(launch_fill(P->d_sp_pass, (uint64_t)n, 0u, P->stream));
     P->sp.assign((size_t)N, SpillShare{});
     P->sp_rowfill.assign((size_t)n, -1);
-    std::vector<int64_t> rowfill((size_t)n);
+    std::vector<int64_t> rowfill((size_t)n), lrp((size_t)n + 1), urp((size_t)n + 1);
     Ctrl snap;
     memset(&snap, 0, sizeof(snap));
     HIPCHK(hipEventRecord(P->ev[0], P->stream));
@@ -1073,13 +1074,11 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares, bool
         HIPCHK(spill_rows(P->d_Lrp, P->d_Lcol, P->d_Urp, n, N, gid, P->d_sp_lcnt, P->d_sp_ucnt, P->d_sp_pass,
                           P->stream));
         SpillShare& h = P->sp[(size_t)gid];
-        h.Lrp.resize((size_t)n + 1);
-        h.Urp.resize((size_t)n + 1);
         h.Lcol.resize((size_t)std::max<int64_t>(P->nnzL, 1));
         h.Ucol.resize((size_t)std::max<int64_t>(P->nnzU, 1));
-        HIPCHK(hipMemcpyAsync(h.Lrp.data(), P->d_Lrp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+        HIPCHK(hipMemcpyAsync(lrp.data(), P->d_Lrp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
                               P->stream));
-        HIPCHK(hipMemcpyAsync(h.Urp.data(), P->d_Urp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+        HIPCHK(hipMemcpyAsync(urp.data(), P->d_Urp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
                               P->stream));
         if (P->nnzL)
             HIPCHK(hipMemcpyAsync(h.Lcol.data(), P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL,
@@ -1090,7 +1089,14 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares, bool
         HIPCHK(hipMemcpyAsync(rowfill.data(), P->d_rowfill, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost,
                               P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
-        for (int32_t i = gid; i < n; i += N) P->sp_rowfill[(size_t)i] = rowfill[(size_t)i];
+        // the share's rows are the only non-empty ones: keep their offsets
+        h.Loff.assign(1, 0);
+        h.Uoff.assign(1, 0);
+        for (int32_t i = gid; i < n; i += N) {
+            P->sp_rowfill[(size_t)i] = rowfill[(size_t)i];
+            h.Loff.push_back(lrp[(size_t)i + 1]);
+            h.Uoff.push_back(urp[(size_t)i + 1]);
+        }
         nnzL += P->nnzL;
         nnzU += P->nnzU;
         snap = P->stats;  // the next share continues the counters with an empty fill stream
@@ -1259,9 +1265,10 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         r->L_rowptr[0] = r->U_rowptr[0] = 0;
         for (int32_t i = 0; i < n; i++) {
             const SpillShare& h = P->sp[(size_t)(i % N)];
-            if (!h.Lrp.empty()) {
-                const int64_t lb = h.Lrp[(size_t)i], le = h.Lrp[(size_t)i + 1];
-                const int64_t ub = h.Urp[(size_t)i], ue = h.Urp[(size_t)i + 1];
+            const size_t k = (size_t)(i / N);  // the row's place in its share
+            if (k + 1 < h.Loff.size()) {
+                const int64_t lb = h.Loff[k], le = h.Loff[k + 1];
+                const int64_t ub = h.Uoff[k], ue = h.Uoff[k + 1];
                 memcpy(r->L_col + lo, h.Lcol.data() + lb, sizeof(int32_t) * (size_t)(le - lb));
                 memcpy(r->U_col + uo, h.Ucol.data() + ub, sizeof(int32_t) * (size_t)(ue - ub));
                 lo += le - lb;
