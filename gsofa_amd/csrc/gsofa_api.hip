@@ -183,6 +183,7 @@ void dfree(T*& p) {
 }
 
 constexpr int32_t kMaxN = (1 << 25) - 1;  // wave-merge totals stay below 2^31
+constexpr int64_t kMaxBlockNnz = 1ll << 30;  // block kernel: column byte offsets below 2^32
 
 int validate(const gsofa_matrix* A) {
     if (!A || A->n < 0 || A->nnz < 0) return GSOFA_ERR_INVALID;
@@ -585,7 +586,7 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     if (opt) o = *opt;
     else gsofa_options_default(&o);
     if (!A || A->n <= 0) return 0;
-    if (o.mode == GSOFA_MODE_GRID) {
+    if (o.mode == GSOFA_MODE_GRID || (o.mode == GSOFA_MODE_AUTO && A->nnz >= kMaxBlockNnz)) {
         Sizing sz;
         if (choose_sizing(A, &o, A->nnz, INT64_MAX / 4, &sz) != GSOFA_OK) return -1;
         return sz.total_bytes;
@@ -661,6 +662,18 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     int64_t avail = (int64_t)((double)freeb * 0.80) - 8 * (int64_t)(n + 1 + P->nnz) -
                     64ll * (n + 1) - (64ll << 20);
     P->mode = P->opt.mode == GSOFA_MODE_GRID ? GSOFA_MODE_GRID : GSOFA_MODE_BLOCK;
+    // The block kernel addresses the packed column array with 32-bit byte
+    // offsets.  Past that, AUTO takes the grid kernel (64-bit addressing);
+    // an explicit block or hybrid request, or a batch, cannot be served.
+    if (P->mode == GSOFA_MODE_BLOCK && P->nnz >= kMaxBlockNnz) {
+        if (P->opt.mode == GSOFA_MODE_AUTO && P->opt.top_sources <= 0 && n_mats == 1) {
+            P->mode = GSOFA_MODE_GRID;
+        } else {
+            (void)hipStreamDestroy(P->stream);
+            delete P;
+            return GSOFA_ERR_TOO_LARGE;
+        }
+    }
     P->orphan_words = (uint32_t)(((uint64_t)n + 31) / 32);
     if (P->mode == GSOFA_MODE_BLOCK && P->opt.top_sources > 0 && n_mats == 1 &&
         P->opt.n_virtual_gpu == 1 && n > 0) {

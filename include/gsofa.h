@@ -47,7 +47,7 @@ const char* gsofa_strerror(int status);
  * Entries must be distinct within a row; the diagonal is expected present. */
 typedef struct gsofa_matrix {
     int32_t n;
-    int64_t nnz; /* length of csr; must be < 2^31 */
+    int64_t nnz; /* length of csr; must be < 2^31 (block mode: < 2^30 after packing) */
     const int32_t* colbeg;
     const int32_t* colend;
     const int32_t* csr;
