@@ -120,6 +120,26 @@ def read_binary(path):
     return _from_csr(c)
 
 
+def order_nd(A, leaf=64):
+    """Nested-dissection ordering of A's symmetrized graph (gsofa_order_nd):
+    perm[k] = original index of the k-th vertex."""
+    m = A.to_c()
+    perm = np.zeros(max(A.n, 1), np.int32)
+    _check(_lib.lib().gsofa_order_nd(C.byref(m), int(leaf), perm.ctypes.data), "order_nd")
+    return perm[:A.n].copy()
+
+
+def permute(A, perm):
+    """B = P A P^T (gsofa_permute): row k of B is row perm[k] of A."""
+    m = A.to_c()
+    p = np.ascontiguousarray(perm, dtype=np.int32)
+    if p.size != A.n:
+        raise GsofaError(-1, "permute: perm has %d entries for n=%d" % (p.size, A.n))
+    c = _lib.Csr()
+    _check(_lib.lib().gsofa_permute(C.byref(m), p.ctypes.data if p.size else None, C.byref(c)), "permute")
+    return _from_csr(c)
+
+
 def write_binary(A, path):
     m = A.to_c()
     _check(_lib.lib().gsofa_write_binary(C.byref(m), path.encode()), "write_binary")

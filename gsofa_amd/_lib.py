@@ -25,6 +25,7 @@ EXPORTS = [
     "gsofa_server_poll", "gsofa_server_wait", "gsofa_server_destroy",
     "gsofa_plan_fetch_supernodal", "gsofa_supernodal_free",
     "gsofa_write_binary", "gsofa_read_binary", "gsofa_map_binary", "gsofa_unmap_binary",
+    "gsofa_order_nd", "gsofa_permute",
 ]
 GSOFA_PENDING = 1
 
@@ -116,6 +117,10 @@ def lib():
     L.gsofa_supernodal_free.argtypes = [P(Supernodal)]
     L.gsofa_plan_footprint.argtypes = [P(Matrix), P(Options)]
     L.gsofa_plan_footprint.restype = C.c_int64
+    L.gsofa_order_nd.argtypes = [P(Matrix), C.c_int32, C.c_void_p]
+    L.gsofa_order_nd.restype = C.c_int
+    L.gsofa_permute.argtypes = [P(Matrix), C.c_void_p, P(Csr)]
+    L.gsofa_permute.restype = C.c_int
     L.gsofa_write_binary.argtypes = [P(Matrix), C.c_char_p]
     L.gsofa_write_binary.restype = C.c_int
     L.gsofa_read_binary.argtypes = [C.c_char_p, P(Csr)]

@@ -281,6 +281,18 @@ int gsofa_read_binary(const char* path, gsofa_csr* out);
 int gsofa_map_binary(const char* path, gsofa_mapped** handle, gsofa_matrix* view);
 void gsofa_unmap_binary(gsofa_mapped* handle);
 
+/* Fill-reducing ordering for inputs that come without one (the reference
+ * takes SuperLU_DIST's ordering as its input; no counterpart there): nested
+ * dissection on breadth-first level structures of the symmetrized graph of A
+ * (George and Liu, SIAM J. Numer. Anal. 15, 1978).  Regions of at most `leaf`
+ * vertices are numbered as they are; separators come after both sides.
+ * perm[k] = original index of the k-th vertex of the new order (n entries,
+ * caller-owned).  gsofa_permute builds B = P A P^T for such a perm: row k of
+ * B is row perm[k] of A, columns renumbered and sorted (free with
+ * gsofa_csr_free); a perm that is not a permutation gives GSOFA_ERR_INVALID. */
+int gsofa_order_nd(const gsofa_matrix* A, int32_t leaf, int32_t* perm);
+int gsofa_permute(const gsofa_matrix* A, const int32_t* perm, gsofa_csr* out);
+
 /* 5-point (nz == 1) or 7-point Laplacian pattern on an nx*ny*nz grid.  nd_leaf
  * > 0 orders it by geometric nested dissection with that leaf size (0 =
  * natural row-major).  With probability extra_prob a row gains one extra
