@@ -9,13 +9,14 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <vector>
 
 #include "gsofa.h"
 
 static int usage(const char* argv0) {
     fprintf(stderr,
             "usage: %s colbeg colend csr N_GPU MEM_GB [max_supernode_size [N_GPU_Node]]\n"
-            "       [--chunk C] [--share G] [--counts-only] [--spill] [--device D]\n",
+            "       [--chunk C] [--share G] [--counts-only] [--spill] [--order-nd LEAF] [--device D]\n",
             argv0);
     return 1;
 }
@@ -25,12 +26,14 @@ int main(int argc, char** argv) {
     int npos = 0;
     gsofa_options opt;
     gsofa_options_default(&opt);
+    int order_leaf = 0;  // > 0: reorder the input by nested dissection first
     for (int i = 1; i < argc; i++) {
         if (!strcmp(argv[i], "--chunk") && i + 1 < argc) opt.chunk = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--share") && i + 1 < argc) opt.virtual_gpu_id = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--device") && i + 1 < argc) opt.device = atoi(argv[++i]);
         else if (!strcmp(argv[i], "--counts-only")) opt.flags |= GSOFA_FLAG_COUNTS_ONLY;
         else if (!strcmp(argv[i], "--spill")) opt.flags |= GSOFA_FLAG_SPILL;
+        else if (!strcmp(argv[i], "--order-nd") && i + 1 < argc) order_leaf = atoi(argv[++i]);
         else if (npos < 7) pos[npos++] = argv[i];
         else return usage(argv[0]);
     }
@@ -48,6 +51,21 @@ int main(int argc, char** argv) {
         return 2;
     }
     gsofa_matrix A = {m.n, m.nnz, m.colbeg, m.colend, m.csr};
+    if (order_leaf > 0) {  // factor P A P^T instead of A
+        std::vector<int32_t> perm((size_t)(m.n > 0 ? m.n : 1));
+        gsofa_csr b;
+        rc = gsofa_order_nd(&A, order_leaf, perm.data());
+        if (!rc) rc = gsofa_permute(&A, perm.data(), &b);
+        if (rc) {
+            fprintf(stderr, "gsofa: ordering failed: %s\n", gsofa_strerror(rc));
+            gsofa_csr_free(&m);
+            return 2;
+        }
+        gsofa_csr_free(&m);
+        m = b;
+        A = {m.n, m.nnz, m.colbeg, m.colend, m.csr};
+        printf("Reordered by nested dissection (leaf %d)\n", order_leaf);
+    }
     gsofa_result r;
     rc = gsofa_symbfact(&A, &opt, &r);
     if (rc) {
