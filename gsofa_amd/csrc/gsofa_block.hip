@@ -17,9 +17,10 @@
 //   (deduplicated: a vertex enters the next queue when its mask leaves zero).
 // * Work is spread over lane groups of SL lanes (two per wave at 32 slots,
 //   eight at 8): lane j of a group always serves slot j.
-// * A level runs in rounds of two phases, and no atomic touches global memory
-//   on the level path (on this part a global atomic executes at the memory
-//   side, not in the L2):
+// * A level runs in rounds of two phases.  The only global atomic on the
+//   level path is one fill-stream reservation per wave and relax batch that
+//   found fills (on this part a global atomic executes at the memory side,
+//   not in the L2); everything else is LDS atomics and plain stores:
 //     A. combine: each lane group pops frontier vertices v, computes lane
 //        j's relaxation cost max(v, label_j(v)) for the slots j of v's mask
 //        (symbfact.cuh:924), looks each neighbour m up in an LDS hash table
