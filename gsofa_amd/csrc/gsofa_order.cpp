@@ -8,7 +8,9 @@
 // problems", SIAM J. Numer. Anal. 15 (1978)), on the symmetrized graph of A
 // (an edge i-j for every entry A(i,j) or A(j,i), i != j):
 //   - a region (a connected set of unnumbered vertices) of at most `leaf`
-//     vertices is numbered as it is (increasing original index);
+//     vertices is numbered by minimum degree on its own elimination graph
+//     (counting its neighbours outside the region, which are separator
+//     vertices numbered later);
 //   - otherwise a breadth-first level structure is rooted at a
 //     pseudo-peripheral vertex (repeated BFS from a minimum-degree vertex of
 //     the last level while the depth grows), the separator is the level where
@@ -165,9 +167,59 @@ private:
         for (size_t c = comps.size(); c-- > 0;) stack_.push_back(Task{std::move(comps[c]), false});
     }
 
+    // A leaf by minimum degree on its own elimination graph: each step
+    // numbers an unnumbered leaf vertex with the fewest neighbours (its leaf
+    // neighbours in the elimination graph plus its original neighbours
+    // outside the leaf, all numbered later), ties to the smaller index, and
+    // joins its leaf neighbours into a clique.
+    void number_leaf(std::vector<int32_t>& v) {
+        const size_t L = v.size();
+        if (L <= 2) {
+            number(v);
+            return;
+        }
+        std::sort(v.begin(), v.end());
+        const int32_t rid = ++rid_;
+        for (int32_t x : v) region_[(size_t)x] = rid;
+        std::vector<std::vector<char>> adj(L, std::vector<char>(L, 0));
+        std::vector<int32_t> ext(L, 0);
+        for (size_t a = 0; a < L; a++)
+            for (int64_t k = G_.xadj[(size_t)v[a]]; k < G_.xadj[(size_t)v[a] + 1]; k++) {
+                const int32_t w = G_.adj[(size_t)k];
+                if (region_[(size_t)w] == rid) {
+                    const size_t b = (size_t)(std::lower_bound(v.begin(), v.end(), w) - v.begin());
+                    adj[a][b] = 1;
+                } else if (region_[(size_t)w] != -2) {
+                    ext[a]++;
+                }
+            }
+        std::vector<char> done(L, 0);
+        for (size_t step = 0; step < L; step++) {
+            size_t best = L;
+            int32_t bestd = 0;
+            for (size_t a = 0; a < L; a++) {
+                if (done[a]) continue;
+                int32_t d = ext[a];
+                for (size_t b = 0; b < L; b++) d += (!done[b] && adj[a][b]) ? 1 : 0;
+                if (best == L || d < bestd) {
+                    best = a;
+                    bestd = d;
+                }
+            }
+            done[best] = 1;
+            perm_[next_++] = v[best];
+            region_[(size_t)v[best]] = -2;
+            for (size_t a = 0; a < L; a++) {
+                if (done[a] || !adj[best][a]) continue;
+                for (size_t b = a + 1; b < L; b++)
+                    if (!done[b] && adj[best][b]) adj[a][b] = adj[b][a] = 1;
+            }
+        }
+    }
+
     void dissect(std::vector<int32_t>& v) {
         if ((int64_t)v.size() <= leaf_) {
-            number(v);
+            number_leaf(v);
             return;
         }
         const int32_t rid = ++rid_;

@@ -285,7 +285,7 @@ void gsofa_unmap_binary(gsofa_mapped* handle);
  * takes SuperLU_DIST's ordering as its input; no counterpart there): nested
  * dissection on breadth-first level structures of the symmetrized graph of A
  * (George and Liu, SIAM J. Numer. Anal. 15, 1978).  Regions of at most `leaf`
- * vertices are numbered as they are; separators come after both sides.
+ * vertices are numbered by minimum degree; separators come after both sides.
  * perm[k] = original index of the k-th vertex of the new order (n entries,
  * caller-owned).  gsofa_permute builds B = P A P^T for such a perm: row k of
  * B is row perm[k] of A, columns renumbered and sorted (free with
