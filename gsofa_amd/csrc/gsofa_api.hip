@@ -1047,7 +1047,7 @@ static int build_supernodes(gsofa_plan* P, const uint32_t* pass) {
 // then its sorted rows go to host memory and the fill stream starts over.
 // Per-row lengths and pass-through flags stay on the device for the
 // supernode pass; the rows themselves are merged at fetch.
-static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares, bool all_rows) {
+static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
     const int32_t n = P->n;
     const int32_t N = std::max(1, P->opt.n_virtual_gpu);
     if (!P->d_sp_lcnt) {
@@ -1124,7 +1124,6 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares, bool
         int rc = build_supernodes(P, P->d_sp_pass);
         if (rc) return rc;
     }
-    (void)all_rows;
     P->traverse_ms = 0;
     P->share_min = 1e300;
     P->share_max = 0;
@@ -1172,7 +1171,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
     }
     if (P->structure && (P->opt.flags & GSOFA_FLAG_SPILL)) {
         P->spilled = true;
-        return execute_spill(P, shares, all_rows);
+        return execute_spill(P, shares);
     }
 
     for (int attempt = 0; attempt < 3; attempt++) {

@@ -100,7 +100,7 @@ private:
     int32_t next_ = 0;
     int32_t rid_ = 0;
     std::vector<int32_t> region_;  // region id of each unnumbered vertex
-    std::vector<int32_t> level_;   // BFS level scratch (valid for the current region id)
+    std::vector<int32_t> level_;   // BFS depth of the vertices the last search reached
     std::vector<Task> stack_;
 
     void number(std::vector<int32_t>& v) {
