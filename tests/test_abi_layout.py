@@ -39,3 +39,14 @@ def test_struct_layouts_match_header(tmp_path):
         assert got[(c, "sizeof")] == ctypes.sizeof(cls), c
         for name, _ in cls._fields_:
             assert got[(c, name)] == getattr(cls, name).offset, (c, name)
+
+
+def test_every_header_function_has_ctypes_argtypes():
+    """Without argtypes, ctypes passes 64-bit pointers as C ints."""
+    import re
+    from gsofa_amd import _lib
+    L = _lib.lib()
+    hdr = open(os.path.join(ROOT, "include", "gsofa.h")).read()
+    names = sorted(set(re.findall(r"\b(gsofa_[a-z0-9_]+)\s*\(", hdr)))
+    assert names
+    assert [n for n in names if getattr(L, n).argtypes is None] == []
