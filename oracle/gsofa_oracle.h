@@ -16,7 +16,10 @@
  * Independent pins (no shared code with the traversal):
  *   orc_brute_lu              boolean Gaussian elimination without pivoting
  *   orc_symbolic_cholesky     elimination-tree column merge (symmetric inputs)
- *   orc_supernodes            the written supernode definition, restated
+ *   orc_supernodes            the written supernode definition, restated; pinned
+ *                             by tests/test_supernode_pins.py (elimination tree
+ *                             + column counts on symmetric inputs, the T2 set
+ *                             form over orc_brute_lu on unsymmetric ones)
  */
 #ifndef GSOFA_ORACLE_H
 #define GSOFA_ORACLE_H

@@ -7,7 +7,8 @@ traversal:
   * textbook boolean LU without pivoting on random unsymmetric patterns;
   * symbolic Cholesky (elimination tree) on structurally symmetric inputs;
   * closed forms for the natural-order 2D grid (|R| = n(n-1)/2, L/U symmetry);
-  * a literal restatement of the supernode definition.
+  * a literal restatement of the supernode definition (the independent
+    supernode pins are in test_supernode_pins.py).
 A planted-mistake test shows the brute-force pin rejects plausible variants
 of the fill rule.
 """
