@@ -54,6 +54,7 @@ struct gsofa_plan {
     BlkChunk* d_chunks = nullptr;
     unsigned long long* d_row_te = nullptr;
     unsigned long long* d_row_reach = nullptr;
+    unsigned long long* d_row_fk = nullptr;   // fills per source row, counted by the block kernel
     uint32_t Q = 0;
     uint32_t nblocks = 0;
     int coop = 1;
@@ -181,9 +182,6 @@ void dfree(T*& p) {
     if (p) (void)hipFree(p);
     p = nullptr;
 }
-
-constexpr int32_t kMaxN = (1 << 25) - 1;  // wave-merge totals stay below 2^31
-constexpr int64_t kMaxBlockNnz = 1ll << 30;  // block kernel: column byte offsets below 2^32
 
 int validate(const gsofa_matrix* A) {
     if (!A || A->n < 0 || A->nnz < 0) return GSOFA_ERR_INVALID;
@@ -432,6 +430,7 @@ void release_device(gsofa_plan* P) {
     dfree(P->d_row_te);
     dfree(P->d_hlab);
     dfree(P->d_row_reach);
+    dfree(P->d_row_fk);
 }
 
 int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
@@ -464,6 +463,8 @@ float ev_ms(hipEvent_t a, hipEvent_t b) {
 }  // namespace
 
 namespace gsofa {
+
+int validate_matrix(const gsofa_matrix* A) { return validate(A); }
 
 // Block mode: per workgroup, labels [W][S] + two masks + two 16-byte-entry
 // queues = (4 S + 40) W bytes (+ 8 of state): 168 W at 32 slots.
@@ -719,6 +720,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         e |= dalloc(&P->d_chunks, std::max<size_t>(P->h_chunks.size(), 1));
         e |= dalloc(&P->d_row_te, (size_t)n);
         e |= dalloc(&P->d_row_reach, (size_t)n);
+        e |= dalloc(&P->d_row_fk, (size_t)n);
         P->lab_words = slots * (size_t)P->W;
         if (P->hybrid) {
             e |= dalloc(&P->d_hlab, (size_t)P->hlab_words);
@@ -942,12 +944,13 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         bp.mat_base = P->d_mat_base;
         bp.row_te = P->d_row_te;
         bp.row_reach = P->d_row_reach;
+        bp.row_fills = P->d_row_fk;
         bp.nblocks = P->nblocks;
         bp.lab = P->d_lab;
         bp.mask = P->d_mask;
         bp.qg = P->d_qgv;
         bp.fills = P->d_fills;
-        bp.fill_cap = P->fill_cap;
+        bp.fill_cap = P->structure ? P->fill_cap : 0;  // counts only: no fill stream
         bp.ctrl = P->d_ctrl;
         bp.bstate = P->d_bstate;
         HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
@@ -1237,8 +1240,8 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->grid_blocks = (int32_t)P->nblocks;
     r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->W : (int64_t)P->Q;
     r->n_super = -1;
-    if (!P->structure || n == 0) {
-        if (n == 0) {
+    if (n == 0) {
+        if (P->structure) {
             r->L_rowptr = (int64_t*)calloc(1, sizeof(int64_t));
             r->U_rowptr = (int64_t*)calloc(1, sizeof(int64_t));
             r->n_super = 0;
@@ -1247,6 +1250,27 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         return GSOFA_OK;
     }
     HIPCHK(hipSetDevice(P->device));
+    if (!P->structure) {
+        // counts only: the block kernel's per-row fills, TE and |R| (grid and
+        // hybrid runs report totals only)
+        if (P->mode != GSOFA_MODE_BLOCK || P->hybrid) return GSOFA_OK;
+        r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+        r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+        r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+        if (!r->row_fill || !r->row_te || !r->row_reach) {
+            gsofa_result_free(r);
+            return GSOFA_ERR_ALLOC;
+        }
+        HIPCHK(hipMemcpyAsync(r->row_fill, P->d_row_fk, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(r->row_te, P->d_row_te, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        const int32_t N = std::max(1, P->opt.n_virtual_gpu), id = P->opt.virtual_gpu_id;
+        for (int32_t i = 0; i < n; i++)
+            if (id >= 0 && i % N != id) r->row_fill[i] = r->row_te[i] = r->row_reach[i] = -1;
+        return GSOFA_OK;
+    }
     r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
     r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
     r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
@@ -1344,18 +1368,30 @@ int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
     r->grid_blocks = G.grid_blocks;
     r->queue_capacity = G.queue_capacity;
     r->n_super = -1;
-    if (!G.L_rowptr || !G.row_fill) return GSOFA_OK;  // counts-only batch
+    if (P->n_mats == 1) {  // the plan's own totals (also for counts-only grid runs)
+        r->fill_count = G.fill_count;
+        r->te_alg = G.te_alg;
+        r->reach = G.reach;
+        r->edge_checks = G.edge_checks;
+        r->levels = G.levels;
+        r->rescans = G.rescans;
+    }
+    if (!G.row_fill) return GSOFA_OK;  // counts only in grid or hybrid mode: totals alone
+    const bool lu = G.L_rowptr != nullptr;  // false: counts only (per-row fills, TE, |R|)
     r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
-    r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
-    r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
-    const int64_t l0 = G.L_rowptr[b0], l1 = G.L_rowptr[b1], u0 = G.U_rowptr[b0], u1 = G.U_rowptr[b1];
-    r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
-    r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
+    const int64_t l0 = lu ? G.L_rowptr[b0] : 0, l1 = lu ? G.L_rowptr[b1] : 0;
+    const int64_t u0 = lu ? G.U_rowptr[b0] : 0, u1 = lu ? G.U_rowptr[b1] : 0;
+    if (lu) {
+        r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+        r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
+        r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
+        r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
+    }
     if (G.row_te) {
         r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
         r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
     }
-    if (!r->row_fill || !r->L_rowptr || !r->U_rowptr || !r->L_col || !r->U_col ||
+    if (!r->row_fill || (lu && (!r->L_rowptr || !r->U_rowptr || !r->L_col || !r->U_col)) ||
         (G.row_te && (!r->row_te || !r->row_reach))) {
         gsofa_result_free(r);
         return GSOFA_ERR_ALLOC;
@@ -1371,15 +1407,18 @@ int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
             reach += (uint64_t)std::max<int64_t>(r->row_reach[i], 0);
         }
     }
+    r->fill_count = fills;
+    if (G.row_te) {  // per-row TE and |R| exist in block mode (every batch)
+        r->te_alg = te;
+        r->reach = reach;
+    }
+    if (!lu) return GSOFA_OK;
     for (int32_t i = 0; i <= nb; i++) {
         r->L_rowptr[i] = G.L_rowptr[b0 + i] - l0;
         r->U_rowptr[i] = G.U_rowptr[b0 + i] - u0;
     }
     for (int64_t k = l0; k < l1; k++) r->L_col[k - l0] = G.L_col[k] - (int32_t)b0;
     for (int64_t k = u0; k < u1; k++) r->U_col[k - u0] = G.U_col[k] - (int32_t)b0;
-    r->fill_count = fills;
-    r->te_alg = te;
-    r->reach = reach;
     if (G.n_super >= 0 && G.supno && nb > 0) {
         // supernodes never cross matrices: (i-1) is never in L_i at a boundary
         const int32_t s0 = G.supno[b0], s1 = G.supno[b1 - 1];

@@ -119,6 +119,7 @@ struct BlkLds {
     uint4 eidx[Z::NG][SL / 4];           // table index of each lane's edge (SL per group)
     unsigned long long s_te[SL];         // per-slot TE of the current mini-group
     unsigned long long s_reach[SL];      // per-slot |R_<(s)|
+    unsigned long long s_fill[SL];       // per-slot fills
     unsigned long long acc[4];           // workgroup totals: fills, TE, |R|, edge checks
     uint32_t src[SL];
     uint32_t qcnt[2];
@@ -467,9 +468,10 @@ __device__ void relax_phase(const Blk& B, BlkLds<SL>& S, uint32_t nbuf, uint32_t
             if (enq) qn[qb + rank_in(em)] = make_uint4(mk, (uint32_t)d0, (uint32_t)d1, 0u);
         }
         // fills of the batch, (source << 32) | target: one reservation per wave
+        // (none when only counts are wanted: fill_cap = 0)
         const uint32_t nf = (uint32_t)__popc(fillbits);
-        if (__ballot(nf != 0u)) {
-            t.fills += nf;
+        t.fills += nf;
+        if (p.fill_cap != 0ull && __ballot(nf != 0u)) {
             const uint32_t incl = wave_incl_scan(nf);
             const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
             uint64_t pos = 0;
@@ -524,6 +526,7 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
             S.src[tid] = tid < nsrc ? C.first - tid * C.stride : 0u;
             S.s_te[tid] = 0;
             S.s_reach[tid] = 0;
+            S.s_fill[tid] = 0;
         }
         if (since >= loops) {  // offsets exhausted: re-initialise this block's labels
             for (uint64_t i = tid; i < (uint64_t)SL * W; i += BB) B.lab0[i] = 0xFFFFFFFFu;
@@ -620,15 +623,17 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
             cur = nbuf;
             levels++;
         }
-        // per-row TE and |R_<(s)|: lane j's tallies belong to slot j
+        // per-row TE, |R_<(s)| and fills: lane j's tallies belong to slot j
         if (t.te) atomicAdd(&S.s_te[j], (unsigned long long)t.te);
         if (t.reach) atomicAdd(&S.s_reach[j], (unsigned long long)t.reach);
-        if (t.fills) atomicAdd(&S.acc[0], (unsigned long long)t.fills);
+        if (t.fills) atomicAdd(&S.s_fill[j], (unsigned long long)t.fills);
         t.te = t.reach = t.fills = 0;
         __syncthreads();
         if (tid < nsrc) {
             p.row_te[S.src[tid]] = S.s_te[tid];
             p.row_reach[S.src[tid]] = S.s_reach[tid];
+            p.row_fills[S.src[tid]] = S.s_fill[tid];
+            atomicAdd(&S.acc[0], S.s_fill[tid]);
             atomicAdd(&S.acc[1], S.s_te[tid]);
             atomicAdd(&S.acc[2], S.s_reach[tid]);
         }

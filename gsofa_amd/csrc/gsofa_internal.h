@@ -35,6 +35,13 @@ struct Ctrl {
     unsigned long long snap_orph[2];
 };
 
+constexpr int32_t kMaxN = (1 << 25) - 1;      // wave-merge totals stay below 2^31
+constexpr int64_t kMaxBlockNnz = 1ll << 30;  // block kernel: column byte offsets below 2^32
+
+// GSOFA_OK, or the status gsofa_plan_create would give for A's shape and
+// index ranges (n, nnz limits; every column index in [0, n)).
+int validate_matrix(const gsofa_matrix* A);
+
 constexpr int kBlockSlots = 32;      // largest source-slot count per workgroup in block mode
 constexpr int kBatchSlots = 16;      // slots per workgroup for batches of matrices by default
 
@@ -63,11 +70,12 @@ struct BlkParams {
     uint32_t* mask;        // nblocks * 2 * W frontier slot masks
     uint4* qg;             // nblocks * 2 * W frontier entries (v, row begin, row end, -)
     uint64_t* fills;
-    uint64_t fill_cap;
+    uint64_t fill_cap;     // 0: count fills without writing the fill stream
     Ctrl* ctrl;
     uint32_t* bstate;      // per workgroup: max-id offset, groups since re-init
     unsigned long long* row_te;     // n: TE of each processed source row
     unsigned long long* row_reach;  // n: |R_<(s)| of each processed source row
+    unsigned long long* row_fills;  // n: fills of each processed source row
 };
 int blk_block_size();
 bool blk_slots_valid(int slots);

@@ -167,12 +167,19 @@ void worker_main(gsofa_server* S) {
         S->cv_work.wait(lk, [&] { return (S->stop || !S->waiting.empty()) && !S->exclusive; });
         if (S->waiting.empty()) break;  // stopping and drained
         std::vector<Request*> batch;
+        int64_t sum_n = 0, sum_nnz = 0;
         while (!S->waiting.empty() && (int32_t)batch.size() < S->opt.max_batch) {
-            batch.push_back(S->waiting.front());
+            Request* r = S->waiting.front();
+            // a batch is one block-diagonal matrix: it keeps the index limits
+            // every request met alone
+            if (sum_n + r->n > gsofa::kMaxN || sum_nnz + r->nnz >= gsofa::kMaxBlockNnz) break;
+            batch.push_back(r);
             if (trav_room(S->share, batch) < 0) {
                 batch.pop_back();
                 break;
             }
+            sum_n += r->n;
+            sum_nnz += r->nnz;
             S->waiting.pop_front();
         }
         int64_t budget = S->share;
@@ -224,13 +231,15 @@ void worker_main(gsofa_server* S) {
 }
 
 int take(gsofa_server* S, Request* r, gsofa_result* out, gsofa_request_info* info) {
-    // caller holds S->mu; r is final
+    // caller holds S->mu; r is final.  Other waiters on the same ticket wake
+    // and find it gone.
     const int st = r->status;
     if (info) *info = r->info;
     if (st == GSOFA_OK && out) *out = r->res;
     else if (st == GSOFA_OK) gsofa_result_free(&r->res);
     S->live.erase(r->ticket);
     delete r;
+    S->cv_done.notify_all();
     return st;
 }
 
@@ -255,7 +264,10 @@ int gsofa_server_create(const gsofa_server_options* opt, gsofa_server** out) {
     gsofa_server_options o;
     if (opt) o = *opt;
     else gsofa_server_options_default(&o);
-    if (o.max_inflight < 1 || o.max_inflight > 16 || o.max_batch < 1) return GSOFA_ERR_INVALID;
+    // spilled output is chosen per request by the server itself (a request
+    // that does not fit runs alone, spilled if it must); the flag is refused
+    if (o.max_inflight < 1 || o.max_inflight > 16 || o.max_batch < 1 || (o.flags & GSOFA_FLAG_SPILL))
+        return GSOFA_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
         (void)hipGetLastError();
@@ -307,6 +319,15 @@ int gsofa_server_submit(gsofa_server* S, const gsofa_matrix* A, int64_t* ticket)
         r->colend[i] = (int32_t)k;
     }
     r->nnz = k;
+    // the checks a plan would make, answered to this caller alone: a
+    // malformed request never reaches a batch shared with others
+    const gsofa_matrix M = {r->n, r->nnz, r->colbeg.data(), r->colend.data(), r->csr.data()};
+    int rc = gsofa::validate_matrix(&M);
+    if (rc == GSOFA_OK && r->nnz >= gsofa::kMaxBlockNnz) rc = GSOFA_ERR_TOO_LARGE;  // batches run in block mode
+    if (rc != GSOFA_OK) {
+        delete r;
+        return rc;
+    }
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) {
@@ -333,11 +354,15 @@ int gsofa_server_poll(gsofa_server* S, int64_t ticket, gsofa_result* out, gsofa_
 int gsofa_server_wait(gsofa_server* S, int64_t ticket, gsofa_result* out, gsofa_request_info* info) {
     if (!S) return GSOFA_ERR_INVALID;
     std::unique_lock<std::mutex> lk(S->mu);
+    // the ticket is looked up afresh on every wake-up: another thread may take
+    // it (poll or a second wait) meanwhile, and then this call fails cleanly
     auto it = S->live.find(ticket);
+    S->cv_done.wait(lk, [&] {
+        it = S->live.find(ticket);
+        return it == S->live.end() || it->second->done;
+    });
     if (it == S->live.end()) return GSOFA_ERR_INVALID;
-    Request* r = it->second;
-    S->cv_done.wait(lk, [&] { return r->done; });
-    return take(S, r, out, info);
+    return take(S, it->second, out, info);
 }
 
 void gsofa_server_destroy(gsofa_server* S) {

@@ -219,8 +219,13 @@ void gsofa_supernodal_free(gsofa_supernodal* s);
  * mem_budget_bytes, runs them as one batch plan (one persistent launch) and
  * publishes every request's result.  Batches of different workers overlap on
  * the device.  Each result equals the request's stand-alone result.
- * A request whose own footprint exceeds budget / max_inflight fails with
- * GSOFA_ERR_BUDGET. */
+ * gsofa_server_submit checks the request as gsofa_plan_create would (index
+ * ranges, size limits) and returns any error to its caller alone.  A request
+ * whose own footprint exceeds budget / max_inflight runs alone once the
+ * device drains, with the whole budget, its output in core or in spilled
+ * shares (gsofa_request_info.spill_shares); only one with room for no
+ * traversal workgroup fails, with GSOFA_ERR_BUDGET.  GSOFA_FLAG_SPILL in the
+ * server's flags is refused (GSOFA_ERR_INVALID): the server chooses it. */
 #define GSOFA_PENDING 1 /* gsofa_server_poll: the request is not finished */
 
 typedef struct gsofa_server gsofa_server;

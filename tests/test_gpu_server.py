@@ -167,3 +167,68 @@ def test_server_poll_and_unknown_ticket():
         _same(got[0], g.symbfact(A))
         with pytest.raises(g.GsofaError):
             srv.poll(t)  # taken
+
+
+def test_server_malformed_request_is_refused_alone():
+    """A request with a column index out of range is refused at submit, to its
+    caller alone; the valid requests around it complete unaffected."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    good = [cf.batch_matrix(20, 1), cf.batch_matrix(33, 2)]
+    bad = g.CsrMatrix.from_rows([[0, 1], [0, 1, 5], [1, 2]])  # 5 >= n = 3
+    neg = g.CsrMatrix.from_rows([[0, -1], [1]])
+    _say("case: server, malformed requests between valid ones")
+    with g.Server(max_inflight=1, max_batch=8) as srv:
+        t0 = srv.submit(good[0])
+        for B in (bad, neg):
+            with pytest.raises(g.GsofaError) as ei:
+                srv.submit(B)
+            assert ei.value.code == -1
+        t1 = srv.submit(good[1])
+        out = [srv.wait(t0)[0], srv.wait(t1)[0]]
+    for A, r in zip(good, out):
+        _same(r, g.symbfact(A))
+
+
+def test_server_two_waiters_on_one_ticket():
+    """Two threads wait on the same ticket: one takes the result, the other
+    gets GSOFA_ERR_INVALID (the ticket is gone), and nothing is freed twice."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    A = cf.batch_matrix(60, 3)
+    got, errs = [], []
+    _say("case: server, two waiters on one ticket")
+    with g.Server(max_inflight=1, max_batch=4) as srv:
+        t = srv.submit(A)
+
+        def waiter():
+            try:
+                got.append(srv.wait(t)[0])
+            except g.GsofaError as e:
+                errs.append(e.code)
+        th = [threading.Thread(target=waiter) for _ in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+    assert len(got) == 1 and errs == [-1]
+    _same(got[0], g.symbfact(A))
+
+
+def test_server_counts_only_matches_oracle():
+    """A counts-only server returns each request's fill count, per-row fills,
+    TE and |R| (no L/U), equal to the oracle's."""
+    import gsofa_amd as g
+    from oracle import oracle
+    reqs = _requests()[:20]
+    _say("case: counts-only server,", len(reqs), "requests")
+    with g.Server(max_inflight=2, max_batch=6, counts_only=True) as srv:
+        ts = [srv.submit(A) for A in reqs]
+        out = [srv.wait(t)[0] for t in ts]
+    for A, r in zip(reqs, out):
+        o = oracle.symbfact(A.colbeg, A.colend, A.csr, A.n)
+        assert r.fill_count == o.fill_count and r.te_alg == o.te_alg and r.reach == o.reach
+        np.testing.assert_array_equal(r.row_fill, o.row_fill)
+        np.testing.assert_array_equal(r.row_te, o.row_te)
+        np.testing.assert_array_equal(r.row_reach, o.row_reach)
+        assert r.L_rowptr is None and r.n_super == -1

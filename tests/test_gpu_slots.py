@@ -151,3 +151,32 @@ def test_auto_slots(bcsstk18):
         p.execute()
         r = p.fetch_matrix(0)
         assert r.chunk // r.grid_blocks == 16
+
+
+@pytest.mark.parametrize("slots", [32, 16, 8])
+def test_counts_only_batch_per_matrix(slots):
+    """A counts-only batch reports every matrix's fill count, per-row fills, TE
+    and |R| (counted by the traversal kernel, no fill stream), equal to the
+    oracle's; a counts-only single plan reports its per-row fills too."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    rng = np.random.default_rng(77 + slots)
+    mats = [cf.batch_matrix(s, b) for b, s in enumerate([16, 31, 48, 70])]
+    mats += [random_matrix(rng, int(rng.integers(1, 300)), 0.02)[0] for _ in range(6)]
+    _say("case: counts-only batch, slots", slots)
+    with g.Plan(mats, counts_only=True, block_slots=slots) as p:
+        p.execute()
+        for b, A in enumerate(mats):
+            r = p.fetch_matrix(b)
+            o = oracle_of(A)
+            assert (r.fill_count, r.te_alg, r.reach) == (o.fill_count, o.te_alg, o.reach), b
+            np.testing.assert_array_equal(r.row_fill, o.row_fill)
+            np.testing.assert_array_equal(r.row_te, o.row_te)
+            np.testing.assert_array_equal(r.row_reach, o.row_reach)
+            assert r.L_rowptr is None
+    A = mats[2]
+    for share in (-1, 1):
+        r = g.symbfact(A, counts_only=True, block_slots=slots, n_virtual_gpu=3, virtual_gpu_id=share)
+        o = oracle_of(A, n_virtual_gpu=3, virtual_gpu_id=share)
+        assert r.fill_count == o.fill_count
+        np.testing.assert_array_equal(r.row_fill, o.row_fill)

@@ -334,6 +334,14 @@ def test_server_argument_checks():
     with pytest.raises(g.GsofaError) as ei:
         g.Server(max_inflight=0)
     assert ei.value.code == -1
+    # spilled output is the server's own choice per request: the flag is refused
+    import ctypes as C
+    from gsofa_amd import _lib
+    o = _lib.ServerOptions()
+    _lib.lib().gsofa_server_options_default(C.byref(o))
+    o.flags = g.GSOFA_FLAG_SPILL
+    h = C.c_void_p()
+    assert _lib.lib().gsofa_server_create(C.byref(o), C.byref(h)) == -1 and not h.value
     try:
         import torch
         has_gpu = torch.cuda.is_available()
