@@ -231,6 +231,7 @@ class SymbfactResult:
         self.levels = int(r.levels)
         self.rescans = int(r.rescans)
         self.n_super = int(r.n_super)
+        self.passes = int(r.passes)
 
         def arr(ptr, cnt, ct, dt):
             if not ptr or cnt <= 0:

@@ -25,7 +25,7 @@ EXPORTS = [
     "gsofa_server_poll", "gsofa_server_wait", "gsofa_server_destroy",
     "gsofa_plan_fetch_supernodal", "gsofa_supernodal_free",
     "gsofa_write_binary", "gsofa_read_binary", "gsofa_map_binary", "gsofa_unmap_binary",
-    "gsofa_order_nd", "gsofa_permute",
+    "gsofa_order_nd", "gsofa_permute", "gsofa_fill_bound",
 ]
 GSOFA_PENDING = 1
 
@@ -60,7 +60,7 @@ class Result(C.Structure):
                 ("share_ms_max", C.c_double), ("chunk", C.c_int32), ("mode", C.c_int32),
                 ("grid_blocks", C.c_int32),
                 ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64),
-                ("row_te", C.c_void_p), ("row_reach", C.c_void_p)]
+                ("row_te", C.c_void_p), ("row_reach", C.c_void_p), ("passes", C.c_int32)]
 
 
 class Supernodal(C.Structure):
@@ -121,6 +121,8 @@ def lib():
     L.gsofa_order_nd.restype = C.c_int
     L.gsofa_permute.argtypes = [P(Matrix), C.c_void_p, P(Csr)]
     L.gsofa_permute.restype = C.c_int
+    L.gsofa_fill_bound.argtypes = [P(Matrix), C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]
+    L.gsofa_fill_bound.restype = C.c_int
     L.gsofa_write_binary.argtypes = [P(Matrix), C.c_char_p]
     L.gsofa_write_binary.restype = C.c_int
     L.gsofa_read_binary.argtypes = [C.c_char_p, P(Csr)]

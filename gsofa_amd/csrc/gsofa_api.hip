@@ -68,6 +68,14 @@ struct gsofa_plan {
     uint64_t* d_fills = nullptr;
     uint64_t fill_cap = 0;
     Ctrl* d_ctrl = nullptr;
+    // Output sized from the row bounds of gsofa_fill_bound: the fill stream
+    // cannot overflow and the L/U columns exist before the traversal, so an
+    // execute runs the traversal once and synchronises with the host only at
+    // its end.  Otherwise (an explicit output_capacity, or bounds that do not
+    // fit the device) the stream grows on demand and an overflow reruns.
+    bool bounded = false;
+    bool totals_known = true;  // nnzL / nnzU hold the last execute's totals
+    int32_t passes = 0;        // traversal passes of the last execute
 
     // hybrid: sources [s_top, n) run first as grid-mode groups [h_g_lo, h_g_hi)
     // of hC sources (markers, orphans and queues below), the rest in block mode
@@ -240,6 +248,35 @@ uint64_t label_words(int32_t n, int32_t N, int32_t C, int32_t gid_lo, int32_t gi
     }
     return best;
 }
+
+// Output one execute needs at most, from the row bounds of lu_row_bounds:
+// fills, L and U entries of the rows it processes (every share, one share, or
+// for a spilled run the largest share).  rp/ci: packed rows.
+void output_need(const gsofa_options& o, int32_t n, const int32_t* rp, const int32_t* ci, uint64_t* need_fill,
+                 uint64_t* need_L, uint64_t* need_U) {
+    std::vector<int64_t> Lb, Ub;
+    lu_row_bounds(n, rp, ci, Lb, Ub);
+    const int32_t N = std::max(1, o.n_virtual_gpu);
+    std::vector<uint64_t> fs((size_t)N, 0), ls((size_t)N, 0), us((size_t)N, 0);
+    for (int32_t i = 0; i < n; i++) {
+        int64_t off = 0;  // A's own entries are not fills
+        for (int32_t q = rp[i]; q < rp[i + 1]; q++) off += ci[q] != i;
+        fs[(size_t)(i % N)] += (uint64_t)(Lb[i] + Ub[i] - off);
+        ls[(size_t)(i % N)] += (uint64_t)Lb[i];
+        us[(size_t)(i % N)] += (uint64_t)Ub[i];
+    }
+    const bool spill = (o.flags & GSOFA_FLAG_SPILL) != 0;
+    *need_fill = *need_L = *need_U = 0;
+    for (int32_t g = 0; g < N; g++) {
+        if (o.virtual_gpu_id >= 0 && g != o.virtual_gpu_id) continue;
+        *need_fill = spill ? std::max(*need_fill, fs[g]) : *need_fill + fs[g];
+        *need_L = spill ? std::max(*need_L, ls[g]) : *need_L + ls[g];
+        *need_U = spill ? std::max(*need_U, us[g]) : *need_U + us[g];
+    }
+}
+
+// Fill stream (8 B per fill) plus sorted and staging columns (8 B per entry).
+int64_t output_bytes(uint64_t fills, uint64_t L, uint64_t U) { return (int64_t)(8 * fills + 8 * (L + U)); }
 
 struct Sizing {
     int32_t C;
@@ -475,9 +512,11 @@ int64_t block_bytes(uint32_t nb, uint32_t W, int slots) {
 // Everything else a block-mode plan allocates: CSR, fill stream, per-row
 // arrays, scan scratch, work table, and the L/U columns (+ sort scratch)
 // reserved for nnz + fill_cap entries.
-int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap) {
-    return 4 * (n + 1) + 4 * nnz + 8 * (int64_t)fill_cap + 96 * n + 8 * (int64_t)scan_tmp_words(n) +
-           16 * (int64_t)nchunks + 4 * ((int64_t)n_mats + 1) + 8 * (nnz + (int64_t)fill_cap) + (1 << 20);
+int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap,
+                          int64_t cols) {
+    if (cols < 0) cols = nnz + (int64_t)fill_cap;
+    return 4 * (n + 1) + 4 * nnz + 8 * (int64_t)fill_cap + 104 * n + 8 * (int64_t)scan_tmp_words(n) +
+           16 * (int64_t)nchunks + 4 * ((int64_t)n_mats + 1) + 8 * cols + (1 << 20);
 }
 
 uint64_t default_fill_cap(int64_t nnz) { return std::max<uint64_t>(64 * (uint64_t)nnz, 1 << 20); }
@@ -600,9 +639,19 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     uint64_t fc = o.output_capacity > 0 ? (uint64_t)o.output_capacity : default_fill_cap(A->nnz);
     if ((o.flags & GSOFA_FLAG_SPILL) && o.output_capacity <= 0)  // one share's fills at a time
         fc = std::max<uint64_t>(fc / (uint64_t)N, 1u << 20);
+    int64_t cols = -1;  // default: room for nnz + fc entries
+    if (o.output_capacity <= 0) {  // what plan creation reserves from the row bounds
+        if (validate(A) != GSOFA_OK) return -1;
+        std::vector<int32_t> rp, ci;
+        pack_rows(A, rp, ci);
+        uint64_t nf = 0, nl = 0, nu = 0;
+        output_need(o, A->n, rp.data(), ci.data(), &nf, &nl, &nu);
+        fc = std::max<uint64_t>(nf, 1);
+        cols = (int64_t)(nl + nu);
+    }
     uint32_t nb = 0;
     if (block_grid(o, S, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
-    return block_bytes(nb, (uint32_t)A->n, S) + block_fixed_bytes(A->n, A->nnz, 1, nchunks, fc);
+    return block_bytes(nb, (uint32_t)A->n, S) + block_fixed_bytes(A->n, A->nnz, 1, nchunks, fc, cols);
 }
 
 }  // extern "C"
@@ -662,6 +711,18 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     (void)hipMemGetInfo(&freeb, &totalb);
     int64_t avail = (int64_t)((double)freeb * 0.80) - 8 * (int64_t)(n + 1 + P->nnz) -
                     64ll * (n + 1) - (64ll << 20);
+    // Output reserve from the row bounds: the fills, L and U entries of the
+    // rows one execute processes (every share, one share, or for a spilled
+    // run the largest share).
+    uint64_t need_fill = 0, need_L = 0, need_U = 0;
+    if (P->opt.output_capacity <= 0 && n > 0) {
+        output_need(P->opt, n, P->h_rp.data(), ci.data(), &need_fill, &need_L, &need_U);
+        const int64_t out_bytes = output_bytes(need_fill, need_L, need_U);
+        if (out_bytes <= avail / 2) {
+            P->bounded = true;
+            avail -= out_bytes;
+        }
+    }
     P->mode = P->opt.mode == GSOFA_MODE_GRID ? GSOFA_MODE_GRID : GSOFA_MODE_BLOCK;
     // The block kernel addresses the packed column array with 32-bit byte
     // offsets.  Past that, AUTO takes the grid kernel (64-bit addressing);
@@ -749,6 +810,15 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     P->fill_cap = P->opt.output_capacity > 0 ? (uint64_t)P->opt.output_capacity : default_fill_cap(P->nnz);
     if ((P->opt.flags & GSOFA_FLAG_SPILL) && P->opt.output_capacity <= 0)  // one share's fills at a time
         P->fill_cap = std::max<uint64_t>(P->fill_cap / (uint64_t)std::max(1, P->opt.n_virtual_gpu), 1u << 20);
+    if (P->bounded) {
+        P->fill_cap = std::max<uint64_t>(need_fill, 1);
+        P->Lcap = (int64_t)need_L;
+        P->Ucap = (int64_t)need_U;
+        e |= dalloc(&P->d_Lcol, (size_t)need_L);
+        e |= dalloc(&P->d_Ltmp, (size_t)need_L);
+        e |= dalloc(&P->d_Ucol, (size_t)need_U);
+        e |= dalloc(&P->d_Utmp, (size_t)need_U);
+    }
     e |= dalloc(&P->d_rp, (size_t)n + 1);
     e |= dalloc(&P->d_ci, (size_t)P->nnz);
     e |= dalloc(&P->d_fills, (size_t)P->fill_cap);
@@ -953,7 +1023,7 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         bp.fill_cap = P->structure ? P->fill_cap : 0;  // counts only: no fill stream
         bp.ctrl = P->d_ctrl;
         bp.bstate = P->d_bstate;
-        HIPCHK(hipMemsetAsync(&P->d_ctrl->src_next, 0, sizeof(uint32_t), P->stream));
+        HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
         HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
         if (P->hybrid && P->h_g_hi > P->h_g_lo) {  // the top groups, whole device each
             TravParams tp = grid_params(P, gid);
@@ -980,15 +1050,19 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
 }
 
 // Sorted L/U rows (d_Lrp/d_Lcol, d_Urp/d_Ucol) of the rows of share gpu_id of
-// N (-1: every row) from the fill stream's n_fills entries.
-static int build_structure(gsofa_plan* P, uint64_t n_fills, int32_t N, int32_t gpu_id) {
+// N (-1: every row) from the fill stream's n_fills entries.  With on_device,
+// the kernels read the stream's length from the traversal's counter and the
+// columns reserved at plan creation hold any result: no host round trip (the
+// totals are read when needed, read_totals).
+static int build_structure(gsofa_plan* P, uint64_t n_fills, int32_t N, int32_t gpu_id, bool on_device = false) {
     const int32_t n = P->n;
     FinalParams f{};
     f.n = n;
     f.rp = P->d_rp;
     f.ci = P->d_ci;
     f.fills = P->d_fills;
-    f.n_fills = n_fills;
+    f.n_fills = on_device ? P->fill_cap : n_fills;
+    f.n_fills_dev = on_device ? &P->d_ctrl->fill_pos : nullptr;
     f.n_gpu = N;
     f.gpu_id = gpu_id;
     f.L_rowptr = P->d_Lrp;
@@ -1000,19 +1074,40 @@ static int build_structure(gsofa_plan* P, uint64_t n_fills, int32_t N, int32_t g
     HIPCHK(launch_count(f, P->stream));
     HIPCHK(exclusive_scan_u32_to_i64(P->d_cntL, P->d_Lrp, n, P->d_scan, P->stream));
     HIPCHK(exclusive_scan_u32_to_i64(P->d_cntU, P->d_Urp, n, P->d_scan, P->stream));
-    int64_t tot[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-    HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-    HIPCHK(hipStreamSynchronize(P->stream));
-    P->nnzL = tot[0];
-    P->nnzU = tot[1];
-    int rc = ensure_cols(P, P->nnzL, P->nnzU);
-    if (rc) return rc;
+    if (on_device) {
+        P->totals_known = false;
+    } else {
+        int64_t tot[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        P->nnzL = tot[0];
+        P->nnzU = tot[1];
+        P->totals_known = true;
+        int rc = ensure_cols(P, P->nnzL, P->nnzU);
+        if (rc) return rc;
+    }
     f.L_col = P->d_Ltmp;
     f.U_col = P->d_Utmp;
+    f.L_cap = P->Lcap;
+    f.U_cap = P->Ucap;
     HIPCHK(launch_scatter(f, P->stream));
-    HIPCHK(launch_sort_segments(P->d_Lrp, P->d_Ltmp, P->d_Lcol, n, P->stream));
-    HIPCHK(launch_sort_segments(P->d_Urp, P->d_Utmp, P->d_Ucol, n, P->stream));
+    HIPCHK(launch_sort_segments(P->d_Lrp, P->d_Ltmp, P->d_Lcol, n, P->Lcap, P->stream));
+    HIPCHK(launch_sort_segments(P->d_Urp, P->d_Utmp, P->d_Ucol, n, P->Ucap, P->stream));
+    return GSOFA_OK;
+}
+
+// nnzL / nnzU of the last structure pass, read from the row pointers' ends.
+static int read_totals(gsofa_plan* P) {
+    if (P->totals_known || P->n == 0) return GSOFA_OK;
+    int64_t tot[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + P->n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + P->n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    if (tot[0] > P->Lcap || tot[1] > P->Ucap) return GSOFA_ERR_STATE;  // cannot happen: the bounds hold
+    P->nnzL = tot[0];
+    P->nnzU = tot[1];
+    P->totals_known = true;
     return GSOFA_OK;
 }
 
@@ -1073,6 +1168,7 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
     for (size_t si = 0; si < shares.size(); si++) {
         const int32_t gid = shares[si];
         for (int attempt = 0; attempt < 3; attempt++) {
+            if (attempt > 0) P->passes++;  // a share reran with a larger stream
             HIPCHK(hipMemcpyAsync(P->d_ctrl, &snap, sizeof(Ctrl), hipMemcpyHostToDevice, P->stream));
             int rc = run_share(P, gid, si);
             if (rc) return rc;
@@ -1143,6 +1239,21 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
     return GSOFA_OK;
 }
 
+// Traversal time per share and in total, and the whole execute's device time,
+// from the events of the last execute (ev[0] .. ev[1]; share si: 2+2si .. 3+2si).
+static void finish_timing(gsofa_plan* P, size_t nshares) {
+    P->traverse_ms = 0;
+    P->share_min = 1e300;
+    P->share_max = 0;
+    for (size_t si = 0; si < nshares; si++) {
+        const double t = ev_ms(P->ev[2 + 2 * si], P->ev[3 + 2 * si]);
+        P->traverse_ms += t;
+        P->share_min = std::min(P->share_min, t);
+        P->share_max = std::max(P->share_max, t);
+    }
+    P->ms = ev_ms(P->ev[0], P->ev[1]);
+}
+
 extern "C" {
 
 int gsofa_plan_execute(gsofa_plan* P) {
@@ -1174,11 +1285,43 @@ int gsofa_plan_execute(gsofa_plan* P) {
     }
     if (P->structure && (P->opt.flags & GSOFA_FLAG_SPILL)) {
         P->spilled = true;
+        P->passes = 1;
         return execute_spill(P, shares);
     }
 
+    P->passes = 0;
+    if (P->bounded) {
+        P->passes = 1;
+        // one pass: traversal, structure and supernodes are queued back to
+        // back; the host waits once, at the end
+        HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_ctrl), sizeof(Ctrl) / 4, 0u, P->stream));
+        HIPCHK(hipEventRecord(P->ev[0], P->stream));
+        for (size_t si = 0; si < shares.size(); si++) {
+            int rc = run_share(P, shares[si], si);
+            if (rc) return rc;
+        }
+        if (P->structure) {
+            int rc = build_structure(P, 0, N, all_rows ? -1 : P->opt.virtual_gpu_id, true);
+            if (!rc && P->supernodes) rc = build_supernodes(P, nullptr);
+            if (rc) return rc;
+        }
+        HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipEventRecord(P->ev[1], P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
+        if (!P->structure || P->stats.fill_pos <= P->fill_cap) {
+            finish_timing(P, shares.size());
+            P->executed = true;
+            return GSOFA_OK;
+        }
+        // the bounds do not hold (cannot happen for a valid matrix): take the
+        // path that grows the stream and reruns
+        P->bounded = false;
+        P->totals_known = true;
+    }
     for (int attempt = 0; attempt < 3; attempt++) {
-        HIPCHK(hipMemsetAsync(P->d_ctrl, 0, sizeof(Ctrl), P->stream));
+        P->passes++;
+        HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_ctrl), sizeof(Ctrl) / 4, 0u, P->stream));
         HIPCHK(hipEventRecord(P->ev[0], P->stream));
         for (size_t si = 0; si < shares.size(); si++) {
             int rc = run_share(P, shares[si], si);
@@ -1194,15 +1337,6 @@ int gsofa_plan_execute(gsofa_plan* P) {
         if (dalloc(&P->d_fills, (size_t)P->fill_cap)) return GSOFA_ERR_ALLOC;
     }
     if (P->structure && P->stats.fill_pos > P->fill_cap) return GSOFA_ERR_ALLOC;
-    P->traverse_ms = 0;
-    P->share_min = 1e300;
-    P->share_max = 0;
-    for (size_t si = 0; si < shares.size(); si++) {
-        double t = ev_ms(P->ev[2 + 2 * si], P->ev[3 + 2 * si]);
-        P->traverse_ms += t;
-        P->share_min = std::min(P->share_min, t);
-        P->share_max = std::max(P->share_max, t);
-    }
 
     if (P->structure) {
         int rc = build_structure(P, P->stats.fill_pos, N, all_rows ? -1 : P->opt.virtual_gpu_id);
@@ -1214,7 +1348,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
     }
     HIPCHK(hipEventRecord(P->ev[1], P->stream));
     HIPCHK(hipStreamSynchronize(P->stream));
-    P->ms = ev_ms(P->ev[0], P->ev[1]);
+    finish_timing(P, shares.size());
     P->executed = true;
     return GSOFA_OK;
 }
@@ -1231,6 +1365,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->edge_checks = P->stats.edge_checks;
     r->levels = P->stats.levels;
     r->rescans = P->stats.rescans;
+    r->passes = P->passes;
     r->ms = P->ms;
     r->traverse_ms = P->traverse_ms;
     r->share_ms_min = P->share_min;
@@ -1250,6 +1385,10 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         return GSOFA_OK;
     }
     HIPCHK(hipSetDevice(P->device));
+    if (P->structure && !P->spilled) {
+        const int rc = read_totals(P);
+        if (rc) return rc;
+    }
     if (!P->structure) {
         // counts only: the block kernel's per-row fills, TE and |R| (grid and
         // hybrid runs report totals only)
@@ -1376,6 +1515,7 @@ int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
         r->levels = G.levels;
         r->rescans = G.rescans;
     }
+    r->passes = G.passes;
     if (!G.row_fill) return GSOFA_OK;  // counts only in grid or hybrid mode: totals alone
     const bool lu = G.L_rowptr != nullptr;  // false: counts only (per-row fills, TE, |R|)
     r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
@@ -1469,6 +1609,8 @@ int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
     h->n = n;
     int32_t ns = 0;
     HIPCHK(hipSetDevice(P->device));
+    const int trc = read_totals(P);
+    if (trc) return trc;
     if (n > 0) {
         HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipStreamSynchronize(P->stream));
@@ -1526,6 +1668,30 @@ int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
         HIPCHK(hipMemcpyAsync(h->sub, d_sub, sizeof(int32_t) * (size_t)nsub, hipMemcpyDeviceToHost, P->stream));
     }
     HIPCHK(hipStreamSynchronize(P->stream));
+    return GSOFA_OK;
+}
+
+int gsofa_fill_bound(const gsofa_matrix* A, int64_t* row_fill_bound, uint64_t* fill_bound, uint64_t* L_bound,
+                     uint64_t* U_bound) {
+    const int rc = validate(A);
+    if (rc) return rc;
+    std::vector<int32_t> rp, ci;
+    pack_rows(A, rp, ci);
+    std::vector<int64_t> Lb, Ub;
+    lu_row_bounds(A->n, rp.data(), ci.data(), Lb, Ub);
+    uint64_t f = 0, l = 0, u = 0;
+    for (int32_t i = 0; i < A->n; i++) {
+        int64_t off = 0;  // A's own off-diagonal entries are not fills
+        for (int32_t p = rp[i]; p < rp[i + 1]; p++) off += ci[p] != i;
+        const int64_t b = Lb[i] + Ub[i] - off;
+        if (row_fill_bound) row_fill_bound[i] = b;
+        f += (uint64_t)b;
+        l += (uint64_t)Lb[i];
+        u += (uint64_t)Ub[i];
+    }
+    if (fill_bound) *fill_bound = f;
+    if (L_bound) *L_bound = l;
+    if (U_bound) *U_bound = u;
     return GSOFA_OK;
 }
 

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "gsofa.h"
 
 namespace gsofa {
@@ -41,6 +43,12 @@ constexpr int64_t kMaxBlockNnz = 1ll << 30;  // block kernel: column byte offset
 // GSOFA_OK, or the status gsofa_plan_create would give for A's shape and
 // index ranges (n, nnz limits; every column index in [0, n)).
 int validate_matrix(const gsofa_matrix* A);
+
+// Per-row upper bounds of |L_i| and |U_i| (strict; A's own entries included)
+// from the elimination tree of A + A^T (gsofa_bound.cpp).  rp/ci are packed
+// rows, sorted and duplicate-free.
+void lu_row_bounds(int32_t n, const int32_t* rp, const int32_t* ci, std::vector<int64_t>& Lb,
+                   std::vector<int64_t>& Ub);
 
 constexpr int kBlockSlots = 32;      // largest source-slot count per workgroup in block mode
 constexpr int kBatchSlots = 16;      // slots per workgroup for batches of matrices by default
@@ -82,7 +90,10 @@ bool blk_slots_valid(int slots);
 int choose_slots(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* slots);
 // Block-mode sizing shared by plan creation, footprints and the server.
 int64_t block_bytes(uint32_t nb, uint32_t W, int slots);
-int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap);
+// cols: L + U column entries reserved (sorted + staging, 8 B each); < 0 means
+// nnz + fill_cap.
+int64_t block_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fill_cap,
+                          int64_t cols = -1);
 uint64_t default_fill_cap(int64_t nnz);
 int block_grid(const gsofa_options& o, int slots, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
 int resident_blocks(int slots, int device, uint32_t* out);
@@ -131,13 +142,15 @@ struct FinalParams {
     const int32_t* rp;
     const int32_t* ci;
     const uint64_t* fills;
-    uint64_t n_fills;
+    uint64_t n_fills;      // entries of the fill stream, or with n_fills_dev:
+    const unsigned long long* n_fills_dev;  // read on the device, at most n_fills
     int32_t n_gpu;
     int32_t gpu_id;        // -1: all rows processed
     int64_t* L_rowptr;     // n+1
     int64_t* U_rowptr;     // n+1
     int32_t* L_col;
     int32_t* U_col;
+    int64_t L_cap, U_cap;  // entries allocated in L_col / U_col: no write beyond
     int64_t* row_fill;     // n
     uint32_t* cntL;        // n scratch (counts, then cursors)
     uint32_t* cntU;
@@ -145,7 +158,9 @@ struct FinalParams {
 };
 hipError_t launch_count(const FinalParams& f, hipStream_t s);
 hipError_t launch_scatter(const FinalParams& f, hipStream_t s);
-hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n,
+// Sorts each segment [rowptr[r], rowptr[r+1]) of src into dst; segments that
+// end beyond cap entries are skipped.
+hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n, int64_t cap,
                                 hipStream_t s);
 
 // Inclusive/exclusive scans over int64 (used for row pointers and supernodes).

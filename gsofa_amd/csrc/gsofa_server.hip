@@ -11,7 +11,8 @@
 // overlap on the device and the queue drains continuously.
 //
 // Admission: each worker owns budget / max_inflight bytes.  A batch's bytes
-// are its fixed arrays (CSR, fill stream, outputs; block_fixed_bytes) plus its
+// are its fixed arrays (CSR, per-row arrays, and the fill stream and output
+// columns sized from the requests' bounds, gsofa_fill_bound; block_fixed_bytes) plus its
 // traversal arrays, which take whatever the fixed part leaves of the share
 // (the plan shrinks its grid to fit, block_grid).  A request is added while at
 // least one workgroup still fits.  One that cannot fit an otherwise empty
@@ -48,6 +49,8 @@ struct Request {
     std::vector<int32_t> colbeg, colend, csr;  // owned copy, rows as submitted
     int32_t n = 0;
     int64_t nnz = 0;
+    uint64_t fill_bound = 0;  // gsofa_fill_bound: fills, and L + U entries
+    uint64_t lu_bound = 0;
     Clock::time_point t_submit, t_start, t_done;
     bool done = false;
     int status = GSOFA_PENDING;
@@ -79,20 +82,26 @@ namespace {
 
 constexpr int32_t kMaxSpill = 256;  // most spilled shares tried for one request
 
-// Bytes the batch `reqs` would need apart from its traversal arrays, with its
-// output in `shares` spilled shares (1: in core; the fill stream then holds
-// one share, as gsofa_plan_create sizes it).
+// Bytes the batch `reqs` would need apart from its traversal arrays: CSR,
+// per-row arrays and the output reserved from the requests' bounds (as
+// gsofa_plan_create reserves it).  With `shares` spilled shares (one request)
+// the device holds one share's output: the cyclic shares' bounds are close to
+// equal, and a quarter more covers the largest.
 int64_t fixed_bytes(const std::vector<Request*>& reqs, int32_t shares = 1) {
     int64_t n = 0, nnz = 0;
-    uint64_t nchunks = 0;
+    uint64_t nchunks = 0, fb = 0, lu = 0;
     for (const Request* r : reqs) {
         n += r->n;
         nnz += r->nnz;
+        fb += r->fill_bound;
+        lu += r->lu_bound;
         nchunks += ((uint64_t)r->n + gsofa::kBatchSlots - 1) / gsofa::kBatchSlots;
     }
-    uint64_t fc = gsofa::default_fill_cap(nnz);
-    if (shares > 1) fc = std::max<uint64_t>(fc / (uint64_t)shares, 1u << 20);
-    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, fc);
+    if (shares > 1) {
+        fb = fb / (uint64_t)shares + fb / (4 * (uint64_t)shares) + 1;
+        lu = lu / (uint64_t)shares + lu / (4 * (uint64_t)shares) + 1;
+    }
+    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, std::max<uint64_t>(fb, 1), (int64_t)lu);
 }
 
 uint32_t largest(const std::vector<Request*>& reqs) {
@@ -321,8 +330,12 @@ int gsofa_server_submit(gsofa_server* S, const gsofa_matrix* A, int64_t* ticket)
     r->nnz = k;
     // the checks a plan would make, answered to this caller alone: a
     // malformed request never reaches a batch shared with others
+    // (gsofa_fill_bound validates as gsofa_plan_create does), and the
+    // request's output bounds, which admission counts against the budget
     const gsofa_matrix M = {r->n, r->nnz, r->colbeg.data(), r->colend.data(), r->csr.data()};
-    int rc = gsofa::validate_matrix(&M);
+    uint64_t lb = 0, ub = 0;
+    int rc = gsofa_fill_bound(&M, nullptr, &r->fill_bound, &lb, &ub);
+    r->lu_bound = lb + ub;
     if (rc == GSOFA_OK && r->nnz >= gsofa::kMaxBlockNnz) rc = GSOFA_ERR_TOO_LARGE;  // batches run in block mode
     if (rc != GSOFA_OK) {
         delete r;

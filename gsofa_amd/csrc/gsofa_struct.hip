@@ -49,8 +49,17 @@ __global__ void k_count_rows(FinalParams f) {
     }
 }
 
+// Fill-stream entries to read: the host's count, or the traversal's append
+// position left in device memory (capped at the stream's length).
+__device__ __forceinline__ uint64_t fills_in_stream(const FinalParams& f) {
+    if (!f.n_fills_dev) return f.n_fills;
+    const uint64_t pos = *f.n_fills_dev;
+    return pos < f.n_fills ? pos : f.n_fills;
+}
+
 __global__ void k_count_fills(FinalParams f) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n_fills;
+    const uint64_t nf = fills_in_stream(f);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = f.fills[i];
         const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
@@ -69,8 +78,13 @@ __global__ void k_scatter_rows(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
             const int64_t pl = f.L_rowptr[s], pu = f.U_rowptr[s];
             for (int32_t k = f.rp[s]; k < f.rp[s + 1]; k++) {
                 const int32_t j = f.ci[k];
-                if (j < s) tmpL[pl + l++] = j;
-                else if (j > s) tmpU[pu + u++] = j;
+                if (j < s) {
+                    if (pl + l < f.L_cap) tmpL[pl + l] = j;
+                    l++;
+                } else if (j > s) {
+                    if (pu + u < f.U_cap) tmpU[pu + u] = j;
+                    u++;
+                }
             }
         }
         f.cntL[s] = l;
@@ -79,12 +93,18 @@ __global__ void k_scatter_rows(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
 }
 
 __global__ void k_scatter_fills(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n_fills;
+    const uint64_t nf = fills_in_stream(f);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = f.fills[i];
         const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
-        if (col < row) tmpL[f.L_rowptr[row] + atomicAdd(&f.cntL[row], 1u)] = (int32_t)col;
-        else tmpU[f.U_rowptr[row] + atomicAdd(&f.cntU[row], 1u)] = (int32_t)col;
+        if (col < row) {
+            const int64_t q = f.L_rowptr[row] + atomicAdd(&f.cntL[row], 1u);
+            if (q < f.L_cap) tmpL[q] = (int32_t)col;
+        } else {
+            const int64_t q = f.U_rowptr[row] + atomicAdd(&f.cntU[row], 1u);
+            if (q < f.U_cap) tmpU[q] = (int32_t)col;
+        }
     }
 }
 
@@ -109,8 +129,11 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t x, uint32_t* sh
 // arrays.  Values within a segment are distinct, so a segment longer than the
 // LDS bitonic tile is sorted by setting bits of an LDS bitmap over successive
 // value windows and emitting them in order.
+// Rows ending beyond cap (entries allocated in src and dst) are left alone,
+// so a row-pointer total above the allocation cannot write out of bounds
+// (the host reports that case as an error).
 __global__ __launch_bounds__(SB) void k_sort_segments(const int64_t* rowptr, const int32_t* src,
-                                                      int32_t* dst, int32_t n) {
+                                                      int32_t* dst, int32_t n, int64_t cap) {
     __shared__ int32_t s[SORT_LDS];
     __shared__ uint32_t bm[BM_WORDS];
     __shared__ uint32_t sh[SB];
@@ -119,7 +142,7 @@ __global__ __launch_bounds__(SB) void k_sort_segments(const int64_t* rowptr, con
     for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
         const int64_t b = rowptr[r], e = rowptr[r + 1];
         const int64_t len = e - b;
-        if (len <= 0) continue;
+        if (len <= 0 || b < 0 || e > cap) continue;
         if (len == 1) {
             if (t == 0) dst[b] = src[b];
             continue;
@@ -413,7 +436,7 @@ hipError_t transpose_structure(const int64_t* rowptr, const int32_t* col, int64_
     hipLaunchKernelGGL(k_col_scatter, grid_for(n, SB / 64), dim3(SB), 0, s, rowptr, col, n, colptr, cnt, tmp);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return launch_sort_segments(colptr, tmp, rows, n, s);
+    return launch_sort_segments(colptr, tmp, rows, n, nnz, s);
 }
 
 // Per-supernode subscripts: counts into cnt (ns), then (after the caller's
@@ -471,10 +494,10 @@ hipError_t launch_scatter(const FinalParams& f, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n,
+hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n, int64_t cap,
                                 hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_segments, grid_for(n, 1, 1 << 20), dim3(SB), 0, s, rowptr, src, dst, n);
+    hipLaunchKernelGGL(k_sort_segments, grid_for(n, 1, 1 << 20), dim3(SB), 0, s, rowptr, src, dst, n, cap);
     return hipGetLastError();
 }
 

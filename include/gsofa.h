@@ -144,9 +144,25 @@ typedef struct gsofa_result {
     uint64_t rescans;     /* queue-overflow drain-and-resume passes                    */
     int64_t* row_te;      /* n: TE of each processed row (block mode only; else NULL)  */
     int64_t* row_reach;   /* n: |R_<(s)| of each processed row (block mode only)       */
+    int32_t passes;       /* traversal passes of the execute: 1, or more when an       */
+                          /* explicit output_capacity overflowed and the run repeated  */
 } gsofa_result;
 
 void gsofa_result_free(gsofa_result* r);
+
+/* Upper bounds of the output, on the host, before any traversal (no
+ * counterpart in the reference, which sizes its pool from MEM_GB and spills
+ * when it overflows, with_space_optimization/symbfact.cuh:2079,2170-2380).
+ * A fill path of A is a path of the graph of A + A^T, so struct(L+U) lies in
+ * the pattern of the Cholesky factor C of A + A^T and its transpose: row i
+ * has at most rc(i) L entries and cc(i) U entries (strict row / column counts
+ * of C, from its elimination tree in near-linear time).
+ *   row_fill_bound  n entries (caller-owned, may be NULL): >= row_fill[i]
+ *   fill_bound      >= fill_count;  L_bound >= nnz(L);  U_bound >= nnz(U)
+ * All equal the exact values when A is structurally symmetric.  Plans size
+ * their fill stream and output from these, so no traversal is repeated. */
+int gsofa_fill_bound(const gsofa_matrix* A, int64_t* row_fill_bound, uint64_t* fill_bound,
+                     uint64_t* L_bound, uint64_t* U_bound);
 
 /* One-shot: upload, run, fetch, release.  Replaces symbfact_min_id
  * (with_space_optimization/symbfact.cuh:1553). */

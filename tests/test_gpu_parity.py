@@ -114,7 +114,27 @@ def test_output_regrowth(bcsstk18):
     import gsofa_amd as g
     _say("case: bcsstk18 output capacity 10")
     r = g.symbfact(bcsstk18, output_capacity=10)
+    assert r.passes == 2  # an explicit capacity that overflows: grown and rerun once
     assert_same_structure(r, oracle_of(bcsstk18))
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_fresh_plan_single_pass(mode):
+    """A 32^3 ND grid with 5% extra entries fills about 200 times its nnz, far
+    above a fixed reserve: sized from the row bounds, a fresh plan traverses
+    once, in one execute with one host synchronisation, bit-exact; reused, it
+    gives the same again."""
+    import gsofa_amd as g
+    A = g.gen_grid(32, 32, 32, nd_leaf=64, extra_prob=0.05, extra_dist=2, seed=6)
+    o = oracle_of(A)
+    assert o.fill_count > 64 * A.nnz
+    _say("case: 32^3 ND fresh plan, mode", mode)
+    with g.Plan(A, mode=mode) as p:
+        for it in range(2):
+            p.execute()
+            r = p.fetch()
+            assert r.passes == 1, it
+            assert_same_structure(r, o)
 
 
 def test_small_grid_and_budget(bcsstk18):
@@ -214,6 +234,7 @@ def test_grid3d_nd_full_golden(mode):
     A = cf.grid3d_nd()
     _say("case: 3D 48^3 ND full size, mode", mode)
     r = g.symbfact(A, mode=mode)
+    assert r.passes == 1  # the stream is sized from the row bounds: one traversal
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
         assert rec[k] == gold[k], k
@@ -230,6 +251,7 @@ def test_band_full_golden():
     A = cf.band()
     _say("case: band n=659,033 full size, block mode")
     r = g.symbfact(A, mode="block")
+    assert r.passes == 1
     rec = record_of(r)
     for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
         assert rec[k] == gold[k], k
