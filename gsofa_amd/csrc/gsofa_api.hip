@@ -76,6 +76,11 @@ struct gsofa_plan {
     bool bounded = false;
     bool totals_known = true;  // nnzL / nnzU hold the last execute's totals
     int32_t passes = 0;        // traversal passes of the last execute
+    // Groups between label re-initialisations: the decreasing offset has room
+    // for 2^32 / n of them (with_space_optimization/symbfact.cuh:1928-1932,
+    // 2388-2407).  GSOFA_OFFSET_SPAN=k in the environment at plan creation
+    // lowers it to k, so tests reach the re-initialisation in a few groups.
+    uint32_t span_cap = 0xFFFFFFFFu;
 
     // hybrid: sources [s_top, n) run first as grid-mode groups [h_g_lo, h_g_hi)
     // of hC sources (markers, orphans and queues below), the rest in block mode
@@ -408,7 +413,7 @@ int size_block_mode(gsofa_plan* P, int64_t avail) {
 void advance_groups(gsofa_plan* P, int32_t gid, int32_t C, uint64_t g_lo, uint64_t g_hi) {
     const uint32_t n = (uint32_t)P->n;
     const uint32_t N = (uint32_t)std::max(1, P->opt.n_virtual_gpu);
-    const uint32_t loops = 0xFFFFFFFFu / n;
+    const uint32_t loops = std::min(0xFFFFFFFFu / n, P->span_cap);
     const uint64_t cnt = share_count(P->n, (int32_t)N, gid);
     for (uint64_t g = g_lo; g < g_hi; g++) {
         const uint64_t k0 = g * (uint64_t)C;
@@ -682,6 +687,10 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         ((P->opt.flags & GSOFA_FLAG_SPILL) && n_mats > 1)) {
         delete P;
         return GSOFA_ERR_INVALID;
+    }
+    if (const char* sp = getenv("GSOFA_OFFSET_SPAN")) {
+        const long v = atol(sp);
+        if (v >= 1 && v < (long)0xFFFFFFFFu) P->span_cap = (uint32_t)v;
     }
     if (P->opt.wave_block_threshold <= 0) P->opt.wave_block_threshold = 4096;
     P->opt.wave_block_threshold = std::min(P->opt.wave_block_threshold, 1 << 20);
@@ -984,6 +993,7 @@ static TravParams grid_params(const gsofa_plan* P, int32_t gid) {
     tp.groups_since_init = P->since;
     tp.lab_hw = P->hw;
     tp.stamp_base = P->epoch * (uint32_t)n;
+    tp.loops = std::min(0xFFFFFFFFu / (uint32_t)std::max(n, 1), P->span_cap);
     tp.spin_ticks = P->spin_ticks;
     return tp;
 }
@@ -1021,6 +1031,7 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         bp.qg = P->d_qgv;
         bp.fills = P->d_fills;
         bp.fill_cap = P->structure ? P->fill_cap : 0;  // counts only: no fill stream
+        bp.loops = std::min(0xFFFFFFFFu / P->W, P->span_cap);
         bp.ctrl = P->d_ctrl;
         bp.bstate = P->d_bstate;
         HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));

@@ -494,7 +494,7 @@ __global__ __launch_bounds__(BB) __attribute__((amdgpu_waves_per_eu(GSOFA_WPE, G
     const uint32_t w = tid / WAVE;
     const uint32_t j = lane % SL;
     const uint32_t W = p.W;
-    const uint32_t loops = 0xFFFFFFFFu / W;
+    const uint32_t loops = p.loops;  // offsets before the labels must be reset (<= 2^32 / W)
     Blk B;
     B.p = &p;
     B.W = W;

@@ -84,6 +84,7 @@ struct BlkParams {
     unsigned long long* row_te;     // n: TE of each processed source row
     unsigned long long* row_reach;  // n: |R_<(s)| of each processed source row
     unsigned long long* row_fills;  // n: fills of each processed source row
+    uint32_t loops;        // mini-groups per workgroup between label re-initialisations
 };
 int blk_block_size();
 bool blk_slots_valid(int slots);
@@ -126,7 +127,7 @@ struct TravParams {
     uint32_t groups_since_init; // groups already run since the last label re-init
     uint64_t lab_hw;       // label words already initialised before this launch
     uint32_t stamp_base;   // marker stamp of source s is stamp_base + s + 1
-    uint32_t pad1;
+    uint32_t loops;        // groups between label re-initialisations
     uint64_t spin_ticks;   // wall-clock ticks a barrier waits before aborting
 };
 

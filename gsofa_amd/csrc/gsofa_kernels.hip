@@ -371,7 +371,7 @@ __global__ __launch_bounds__(BLOCK) void k_traverse(TravParams p) {
     unsigned long long levels = 0, rescans = 0;
     const uint32_t n = (uint32_t)p.n;
     const uint32_t N = (uint32_t)p.n_gpu;
-    const uint32_t loops = 0xFFFFFFFFu / n;
+    const uint32_t loops = p.loops;  // groups before the labels must be reset (<= 2^32 / n)
     const uint64_t share_cnt = ((uint64_t)n - (uint64_t)p.gpu_id + N - 1) / N;
     uint32_t off = p.off_start;
     uint32_t since = p.groups_since_init;
