@@ -204,7 +204,7 @@ def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
     o.output_capacity = output_capacity
     o.flags = (GSOFA_FLAG_COUNTS_ONLY if counts_only else 0) | (GSOFA_FLAG_SPILL if spill else 0)
     o.grid_blocks = grid_blocks
-    o.mode = {"auto": 0, "block": 1, "grid": 2}.get(mode, mode)
+    o.mode = {"auto": 0, "block": 1, "grid": 2, "reach": 4}.get(mode, mode)
     o.top_sources = top_sources
     o.block_slots = block_slots
     return o
@@ -225,7 +225,7 @@ class SymbfactResult:
         self.share_ms_min = float(r.share_ms_min)
         self.share_ms_max = float(r.share_ms_max)
         self.chunk = int(r.chunk)
-        self.mode = {1: "block", 2: "grid", 3: "hybrid"}.get(int(r.mode), int(r.mode))
+        self.mode = {1: "block", 2: "grid", 3: "hybrid", 4: "reach"}.get(int(r.mode), int(r.mode))
         self.grid_blocks = int(r.grid_blocks)
         self.queue_capacity = int(r.queue_capacity)
         self.levels = int(r.levels)

@@ -55,6 +55,16 @@ struct gsofa_plan {
     unsigned long long* d_row_te = nullptr;
     unsigned long long* d_row_reach = nullptr;
     unsigned long long* d_row_fk = nullptr;   // fills per source row, counted by the block kernel
+    // reach mode: A^T as packed rows, visited words and next-level masks per
+    // workgroup (bits = sources per workgroup), per-matrix TE and |R|
+    int32_t* d_rpT = nullptr;
+    int32_t* d_ciT = nullptr;
+    void* d_vis = nullptr;
+    void* d_rmask = nullptr;
+    int rbits = 0;
+    unsigned long long* d_mat_te = nullptr;
+    unsigned long long* d_mat_reach = nullptr;
+    std::vector<unsigned long long> h_mat_te, h_mat_reach;  // last fetch (batches)
     uint32_t Q = 0;
     uint32_t nblocks = 0;
     int coop = 1;
@@ -473,6 +483,12 @@ void release_device(gsofa_plan* P) {
     dfree(P->d_hlab);
     dfree(P->d_row_reach);
     dfree(P->d_row_fk);
+    dfree(P->d_rpT);
+    dfree(P->d_ciT);
+    dfree(P->d_vis);
+    dfree(P->d_rmask);
+    dfree(P->d_mat_te);
+    dfree(P->d_mat_reach);
 }
 
 int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
@@ -594,6 +610,66 @@ int block_grid(const gsofa_options& o, int S, uint64_t nchunks, uint32_t W, int6
     return GSOFA_OK;
 }
 
+// Reach mode: sources per workgroup (32 or 64).  Unless the options fix it,
+// a single matrix takes 64 when its chunks still give every co-resident
+// workgroup one, else 32; batches take 64.
+int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits) {
+    if (o.block_slots != 0) {
+        if (!rch_bits_valid(o.block_slots)) return GSOFA_ERR_INVALID;
+        *bits = o.block_slots;
+        return GSOFA_OK;
+    }
+    if (n_mats > 1) {
+        *bits = 64;
+        return GSOFA_OK;
+    }
+    uint32_t nb = 0;
+    int rc = reach_resident(o.device, &nb);
+    if (rc) return rc;
+    *bits = (nsrc + 63) / 64 >= nb ? 64 : 32;
+    return GSOFA_OK;
+}
+
+int reach_resident(int device, uint32_t* out) {
+    int occ = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, rch_kernel_ptr(64), rch_block_size(), 0) != hipSuccess ||
+        occ <= 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
+        (void)hipGetLastError();
+        return GSOFA_ERR_HIP;
+    }
+    *out = (uint32_t)occ * (uint32_t)cus;
+    return GSOFA_OK;
+}
+
+// As block_grid, for the reach kernel's per-workgroup arrays.
+int reach_grid(const gsofa_options& o, int bits, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out) {
+    if (o.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, o.mem_budget_bytes);
+    uint32_t nb = 0;
+    const int rc = reach_resident(o.device, &nb);
+    if (rc) return rc;
+    if (o.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)o.grid_blocks);
+    if (o.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, o.chunk / bits));
+    nb = (uint32_t)std::min<uint64_t>(nb, std::max<uint64_t>(nchunks, 1));
+    while (reach_bytes(nb, W, bits) > avail && nb > 1) nb = nb > 64 ? nb - nb / 8 : nb - 1;
+    if (reach_bytes(nb, W, bits) > avail) return GSOFA_ERR_BUDGET;
+    *nb_out = nb;
+    return GSOFA_OK;
+}
+
+// Packed rows of A^T from packed rows of A (sorted, as counting sort keeps
+// the row order of A within each column).
+void transpose_rows(int32_t n, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& rpT,
+                    std::vector<int32_t>& ciT) {
+    rpT.assign((size_t)n + 1, 0);
+    const int64_t nnz = rp[n];
+    for (int64_t k = 0; k < nnz; k++) rpT[(size_t)ci[k] + 1]++;
+    for (int32_t i = 0; i < n; i++) rpT[(size_t)i + 1] += rpT[(size_t)i];
+    ciT.resize((size_t)nnz);
+    std::vector<int32_t> cur(rpT.begin(), rpT.end() - 1);
+    for (int32_t i = 0; i < n; i++)
+        for (int32_t k = rp[i]; k < rp[i + 1]; k++) ciT[(size_t)cur[(size_t)ci[k]]++] = i;
+}
+
 }  // namespace gsofa
 
 extern "C" {
@@ -636,6 +712,25 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
         if (choose_sizing(A, &o, A->nnz, INT64_MAX / 4, &sz) != GSOFA_OK) return -1;
         return sz.total_bytes;
     }
+    if (o.mode == GSOFA_MODE_REACH) {
+        if (o.block_slots != 0 && !rch_bits_valid(o.block_slots)) return -1;
+        if (validate(A) != GSOFA_OK) return -1;
+        int bits = 0;
+        if (choose_bits(o, 1, (uint64_t)A->n, &bits) != GSOFA_OK) return -1;
+        const uint64_t nchunks = ((uint64_t)A->n + bits - 1) / bits;
+        std::vector<int32_t> rp, ci;
+        pack_rows(A, rp, ci);
+        uint64_t nf = 0, nl = 0, nu = 0;
+        gsofa_options w = o;
+        w.n_virtual_gpu = 1;
+        w.virtual_gpu_id = -1;
+        output_need(w, A->n, rp.data(), ci.data(), &nf, &nl, &nu);
+        uint32_t nb = 0;
+        if (reach_grid(o, bits, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
+        return reach_bytes(nb, (uint32_t)A->n, bits) +
+               block_fixed_bytes(A->n, (int64_t)ci.size(), 1, nchunks, std::max<uint64_t>(nf, 1), (int64_t)(nl + nu)) +
+               4 * (int64_t)(A->n + 1) + 4 * (int64_t)ci.size();
+    }
     const int32_t N = std::max(1, o.n_virtual_gpu);
     if (o.block_slots != 0 && !blk_slots_valid(o.block_slots)) return -1;
     int S = 0;
@@ -676,17 +771,26 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     P->h_mat_base.assign(mat_base, mat_base + n_mats + 1);
     P->W = 1;
     for (int32_t b = 0; b < n_mats; b++) P->W = std::max<uint32_t>(P->W, mat_base[b + 1] - mat_base[b]);
-    if (n_mats > 1) {  // batches: one share, block mode
+    if (n_mats > 1) {  // batches: one share, block (or reach) mode
         P->opt.n_virtual_gpu = 1;
         P->opt.virtual_gpu_id = -1;
-        P->opt.mode = GSOFA_MODE_BLOCK;
+        if (P->opt.mode != GSOFA_MODE_REACH) P->opt.mode = GSOFA_MODE_BLOCK;
     }
     if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
+    const bool reach = P->opt.mode == GSOFA_MODE_REACH;
     if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu ||
-        (P->opt.block_slots != 0 && !blk_slots_valid(P->opt.block_slots)) ||
-        ((P->opt.flags & GSOFA_FLAG_SPILL) && n_mats > 1)) {
+        (P->opt.block_slots != 0 && !(reach ? rch_bits_valid(P->opt.block_slots)
+                                            : blk_slots_valid(P->opt.block_slots))) ||
+        ((P->opt.flags & GSOFA_FLAG_SPILL) && n_mats > 1) ||
+        // reach mode answers whole-matrix runs in core only
+        (reach && ((P->opt.virtual_gpu_id >= 0 && P->opt.n_virtual_gpu > 1) || (P->opt.flags & GSOFA_FLAG_SPILL) ||
+                   P->opt.top_sources > 0))) {
         delete P;
         return GSOFA_ERR_INVALID;
+    }
+    if (reach) {  // every share in one execute is the whole matrix
+        P->opt.n_virtual_gpu = 1;
+        P->opt.virtual_gpu_id = -1;
     }
     if (const char* sp = getenv("GSOFA_OFFSET_SPAN")) {
         const long v = atol(sp);
@@ -732,7 +836,9 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
             avail -= out_bytes;
         }
     }
-    P->mode = P->opt.mode == GSOFA_MODE_GRID ? GSOFA_MODE_GRID : GSOFA_MODE_BLOCK;
+    P->mode = P->opt.mode == GSOFA_MODE_GRID ? GSOFA_MODE_GRID
+              : reach                          ? GSOFA_MODE_REACH
+                                               : GSOFA_MODE_BLOCK;
     // The block kernel addresses the packed column array with 32-bit byte
     // offsets.  Past that, AUTO takes the grid kernel (64-bit addressing);
     // an explicit block or hybrid request, or a batch, cannot be served.
@@ -769,7 +875,38 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         return GSOFA_ERR_HIP;
     }
     int e = 0;
-    if (P->mode == GSOFA_MODE_BLOCK) {
+    std::vector<int32_t> rpT, ciT;
+    if (P->mode == GSOFA_MODE_REACH) {
+        int bits = 0;
+        rc = choose_bits(P->opt, n_mats, (uint64_t)n, &bits);
+        P->S = (uint32_t)bits;
+        P->rbits = bits;
+        if (!rc) {
+            build_chunks(P);
+            transpose_rows(n, P->h_rp.data(), ci.data(), rpT, ciT);
+            const int64_t tbytes = 4 * (int64_t)(n + 1) + 4 * P->nnz + 16 * (int64_t)n_mats;
+            rc = reach_grid(P->opt, bits, std::max<uint64_t>(P->h_chunks.size(), 1), P->W,
+                            std::max<int64_t>(avail - tbytes, 1 << 20), &P->nblocks);
+        }
+        if (rc) {
+            (void)hipStreamDestroy(P->stream);
+            delete P;
+            return rc;
+        }
+        P->C = (int32_t)(P->nblocks * P->S);
+        const size_t nbw = (size_t)P->nblocks * (size_t)P->W;
+        e |= dalloc((uint64_t**)&P->d_vis, nbw * (bits == 64 ? 2 : 1));
+        e |= dalloc((uint32_t**)&P->d_rmask, 2 * nbw * (size_t)(bits / 32));
+        e |= dalloc(&P->d_qgv, 2 * nbw);
+        e |= dalloc(&P->d_bstate, (size_t)P->nblocks);
+        e |= dalloc(&P->d_mat_base, (size_t)n_mats + 1);
+        e |= dalloc(&P->d_chunks, std::max<size_t>(P->h_chunks.size(), 1));
+        e |= dalloc(&P->d_row_fk, (size_t)n);
+        e |= dalloc(&P->d_mat_te, (size_t)n_mats);
+        e |= dalloc(&P->d_mat_reach, (size_t)n_mats);
+        e |= dalloc(&P->d_rpT, (size_t)n + 1);
+        e |= dalloc(&P->d_ciT, (size_t)P->nnz);
+    } else if (P->mode == GSOFA_MODE_BLOCK) {
         int S = 0;
         const uint64_t nsrc = P->hybrid ? P->s_top : share_count(n, P->opt.n_virtual_gpu, 0);
         rc = choose_slots(P->opt, n_mats, nsrc, &S);
@@ -859,6 +996,23 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     if (P->mode == GSOFA_MODE_BLOCK) {
         ok = ok && hipMemsetAsync(P->d_mask, 0, sizeof(uint32_t) * (size_t)P->nblocks * 2 * P->W,
                                   P->stream) == hipSuccess;
+    }
+    if (P->mode == GSOFA_MODE_REACH) {
+        const size_t nbw = (size_t)P->nblocks * (size_t)P->W;
+        const size_t vb = nbw * (P->rbits == 64 ? 16 : 8), mb = 2 * nbw * (size_t)(P->rbits / 8);
+        ok = ok && hipMemsetAsync(P->d_vis, 0, vb, P->stream) == hipSuccess;
+        ok = ok && hipMemsetAsync(P->d_rmask, 0, mb, P->stream) == hipSuccess;
+        ok = ok && hipMemsetAsync(P->d_bstate, 0, sizeof(uint32_t) * P->nblocks, P->stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(P->d_rpT, rpT.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice,
+                                  P->stream) == hipSuccess;
+        if (P->nnz)
+            ok = ok && hipMemcpyAsync(P->d_ciT, ciT.data(), sizeof(int32_t) * P->nnz, hipMemcpyHostToDevice,
+                                      P->stream) == hipSuccess;
+        ok = ok && hipMemcpyAsync(P->d_mat_base, P->h_mat_base.data(), sizeof(uint32_t) * (n_mats + 1),
+                                  hipMemcpyHostToDevice, P->stream) == hipSuccess;
+        if (!P->h_chunks.empty())
+            ok = ok && hipMemcpyAsync(P->d_chunks, P->h_chunks.data(), sizeof(BlkChunk) * P->h_chunks.size(),
+                                      hipMemcpyHostToDevice, P->stream) == hipSuccess;
     }
     std::vector<uint32_t> bst;
     if (P->mode == GSOFA_MODE_BLOCK) {
@@ -1013,6 +1167,42 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         }
     }
     P->marks_clean = false;
+    if (P->mode == GSOFA_MODE_REACH) {
+        RchParams rp{};
+        rp.rp = P->d_rp;
+        rp.ci = P->d_ci;
+        rp.W = P->W;
+        rp.chunks = P->d_chunks;
+        rp.n_chunks = (uint32_t)P->h_chunks.size();
+        rp.mat_base = P->d_mat_base;
+        rp.nblocks = P->nblocks;
+        rp.vis = P->d_vis;
+        rp.nmask = P->d_rmask;
+        rp.q = P->d_qgv;
+        rp.fills = P->d_fills;
+        rp.fill_cap = P->structure ? P->fill_cap : 0;  // counts only: per-row tallies, no fill stream
+        rp.ctrl = P->d_ctrl;
+        rp.bstate = P->d_bstate;
+        rp.row_fills = P->d_row_fk;
+        rp.mat_te = P->d_mat_te;
+        rp.mat_reach = P->d_mat_reach;
+        rp.loops = std::min(0xFFFFFFFFu, P->span_cap);
+        HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_mat_te), 2 * (uint64_t)P->n_mats, 0u, P->stream));
+        HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_mat_reach), 2 * (uint64_t)P->n_mats, 0u, P->stream));
+        HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
+        HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+        if (rp.n_chunks > 0) {
+            // over A: U rows, TE, |R|; then over A^T: L rows
+            HIPCHK(launch_reach(rp, P->rbits, P->stream));
+            HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
+            rp.rp = P->d_rpT;
+            rp.ci = P->d_ciT;
+            rp.transposed = 1;
+            HIPCHK(launch_reach(rp, P->rbits, P->stream));
+        }
+        HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+        return GSOFA_OK;
+    }
     if (P->mode == GSOFA_MODE_BLOCK) {
         BlkParams bp{};
         bp.rp = P->d_rp;
@@ -1250,6 +1440,20 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
     return GSOFA_OK;
 }
 
+// Reach mode, batches: per-matrix TE and |R| of the last execute to the host
+// (queued on the plan's stream; valid after the next synchronisation).
+static hipError_t fetch_mat_totals(gsofa_plan* P) {
+    if (P->mode != GSOFA_MODE_REACH || P->n_mats <= 1) return hipSuccess;
+    P->h_mat_te.resize((size_t)P->n_mats);
+    P->h_mat_reach.resize((size_t)P->n_mats);
+    hipError_t e = hipMemcpyAsync(P->h_mat_te.data(), P->d_mat_te, sizeof(unsigned long long) * P->n_mats,
+                                  hipMemcpyDeviceToHost, P->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(P->h_mat_reach.data(), P->d_mat_reach, sizeof(unsigned long long) * P->n_mats,
+                           hipMemcpyDeviceToHost, P->stream);
+    return e;
+}
+
 // Traversal time per share and in total, and the whole execute's device time,
 // from the events of the last execute (ev[0] .. ev[1]; share si: 2+2si .. 3+2si).
 static void finish_timing(gsofa_plan* P, size_t nshares) {
@@ -1384,7 +1588,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->chunk = P->C;
     r->mode = P->hybrid ? GSOFA_MODE_HYBRID : P->mode;
     r->grid_blocks = (int32_t)P->nblocks;
-    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK ? (int64_t)P->W : (int64_t)P->Q;
+    r->queue_capacity = P->mode == GSOFA_MODE_BLOCK || P->mode == GSOFA_MODE_REACH ? (int64_t)P->W : (int64_t)P->Q;
     r->n_super = -1;
     if (n == 0) {
         if (P->structure) {
@@ -1401,8 +1605,17 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         if (rc) return rc;
     }
     if (!P->structure) {
-        // counts only: the block kernel's per-row fills, TE and |R| (grid and
-        // hybrid runs report totals only)
+        // counts only: the block kernel's per-row fills, TE and |R|; the
+        // reach kernels' per-row fills (grid and hybrid runs report totals only)
+        if (P->mode == GSOFA_MODE_REACH) {
+            r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+            if (!r->row_fill) return GSOFA_ERR_ALLOC;
+            HIPCHK(hipMemcpyAsync(r->row_fill, P->d_row_fk, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                                  P->stream));
+            HIPCHK(fetch_mat_totals(P));
+            HIPCHK(hipStreamSynchronize(P->stream));
+            return GSOFA_OK;
+        }
         if (P->mode != GSOFA_MODE_BLOCK || P->hybrid) return GSOFA_OK;
         r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
         r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
@@ -1478,6 +1691,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         for (int32_t i = 0; i < n; i++)
             if (r->row_fill[i] < 0) r->row_te[i] = r->row_reach[i] = -1;
     }
+    HIPCHK(fetch_mat_totals(P));
     if (P->supernodes) {
         int32_t ns = 0;
         HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
@@ -1562,6 +1776,9 @@ int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
     if (G.row_te) {  // per-row TE and |R| exist in block mode (every batch)
         r->te_alg = te;
         r->reach = reach;
+    } else if (P->mode == GSOFA_MODE_REACH && P->n_mats > 1 && (size_t)index < P->h_mat_te.size()) {
+        r->te_alg = P->h_mat_te[(size_t)index];  // reach mode: per-matrix totals
+        r->reach = P->h_mat_reach[(size_t)index];
     }
     if (!lu) return GSOFA_OK;
     for (int32_t i = 0; i <= nb; i++) {

@@ -35,6 +35,11 @@ struct Ctrl {
     // (slot gen & 1): queue size of the next phase and outstanding orphans
     unsigned long long snap_q[2];
     unsigned long long snap_orph[2];
+    // reach mode: TE and |R| of the A^T pass, and frontier vertices expanded
+    // (both passes)
+    unsigned long long te_t;
+    unsigned long long reach_t;
+    unsigned long long expansions;
 };
 
 constexpr int32_t kMaxN = (1 << 25) - 1;      // wave-merge totals stay below 2^31
@@ -100,6 +105,40 @@ int block_grid(const gsofa_options& o, int slots, uint64_t nchunks, uint32_t W, 
 int resident_blocks(int slots, int device, uint32_t* out);
 const void* blk_kernel_ptr(int slots);
 hipError_t launch_traverse_blk(const BlkParams& p, int slots, size_t lds_bytes, hipStream_t s);
+
+// Reach mode (gsofa_reach.hip): the same work table, run once over A (U rows,
+// TE, |R|) and once over A^T (L columns, written as (row, column) fills).
+struct RchParams {
+    const int32_t* rp;     // packed row pointers of A, or of A^T in the transposed pass
+    const int32_t* ci;
+    uint32_t W;            // per-workgroup window: largest matrix size
+    const BlkChunk* chunks;
+    uint32_t n_chunks;
+    const uint32_t* mat_base;
+    uint32_t nblocks;
+    void* vis;             // nblocks * W visited words (8 B at 32 slots, 16 B at 64)
+    void* nmask;           // nblocks * 2 * W next-level slot masks (4 or 8 B)
+    uint4* q;              // nblocks * 2 * W frontier entries (v, row begin, row end, -)
+    uint64_t* fills;
+    uint64_t fill_cap;     // 0: count fills per row without writing the fill stream
+    Ctrl* ctrl;
+    uint32_t* bstate;      // per workgroup: stamp of its last mini-group
+    unsigned long long* row_fills;  // counts only: fills per row (the A pass stores, A^T adds)
+    unsigned long long* mat_te;     // per matrix TE and |R| (A pass), or null
+    unsigned long long* mat_reach;
+    uint32_t loops;        // stamps per workgroup before its visited words are cleared
+    uint32_t transposed;   // 1: the A^T pass
+};
+int rch_block_size();
+bool rch_bits_valid(int bits);
+const void* rch_kernel_ptr(int bits);
+int64_t reach_bytes(uint32_t nb, uint32_t W, int bits);
+hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s);
+int reach_resident(int device, uint32_t* out);
+int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits);
+int reach_grid(const gsofa_options& o, int bits, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
+void transpose_rows(int32_t n, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& rpT,
+                    std::vector<int32_t>& ciT);
 
 struct TravParams {
     const int32_t* rp;     // n+1 packed row pointers

@@ -1,0 +1,426 @@
+// gsofa_reach.hip -- two-pass reachability traversal (GSOFA_MODE_REACH).
+//
+// The fill-path theorem splits row s of struct(L+U) into two reachability
+// questions that need no labels:
+//   U_s = {m > s : m is reached from s through vertices below s}, and with it
+//         R_<(s) (the vertices below s so reached) and TE_alg;
+//   L_s = {m < s : s is reached from m through vertices below m in A^T}, since
+//         a path s -> ... -> m of A whose intermediates lie below m is a path
+//         m -> ... -> s of A^T whose intermediates lie below its source m.
+// So one kernel answers both: run over A it yields U rows, |R| and TE exactly
+// as the reference's traversal does for U (with_space_optimization/
+// symbfact.cuh:883-1082, the "m > src" branch at :992-1009); run over A^T it
+// yields L columns, written to the fill stream as (row, column) pairs so the
+// structure pass builds the same sorted rows.  Each (vertex, source) pair is
+// expanded at most once, where the reference's max-id labels re-expand a
+// vertex each time its label drops (5-9 times per pair on 3D nested-dissection
+// grids).
+//
+// Execution model (CDNA4, one 1024-thread workgroup per CU, persistent grid):
+// * A workgroup owns MB source slots (32 or 64: the bits of a mask word) and
+//   runs a mini-group of up to MB sources of one matrix through all of their
+//   levels alone; the next mini-group comes from one global counter.
+// * Per vertex, one word holds the MB slots' visited bits with the
+//   mini-group's stamp beside it; an older stamp reads as "nothing visited",
+//   so the array is never cleared between mini-groups (the reference's
+//   source-stamped markers, symbfact.cuh:2126, generalised to a bit per slot).
+// * A level runs in rounds of two phases separated by workgroup barriers:
+//     A. combine: lane groups of G lanes pop frontier vertices; lane j of a
+//        group takes edge j of the current group of G edges, finds or inserts
+//        the target in an LDS hash table of the round's distinct targets and
+//        ORs the vertex's slot mask into the target's mask (ds_or);
+//     B. relax: one lane per table entry: new = mask & ~visited; the new bits
+//        of slots whose source lies above the target push it to the next
+//        frontier (its R_<(s) membership), the others are U entries (fills
+//        from level 1 on: level 0 expands the sources themselves, whose
+//        neighbours are A's own entries).
+//   Targets are unique within a round, so the visited word, the next-level
+//   mask and the queue append need no global atomics.
+// * Sources of a mini-group form an arithmetic sequence first, first-stride,
+//   ... (the cyclic source map of symbfact.cuh:1084-1099 within a share), so
+//   "source j lies above m" is the prefix of slots j < ceil((first-m)/stride).
+#include "gsofa_device.h"
+#include "gsofa_internal.h"
+
+namespace gsofa {
+
+namespace {
+
+using namespace dev;
+
+constexpr int RB = 1024;              // threads per workgroup
+constexpr int G = 8;                  // lanes per frontier vertex in combine
+constexpr int HB = 13;                // log2 hash slots
+constexpr int H = 1 << HB;
+constexpr int T = H / 2;              // inserts allowed per round (load <= 1/2 + RB/H)
+constexpr uint32_t FULLI = 0xFFFFFFFFu;
+static_assert(T + RB <= H, "insert overshoot stays within the entry list");
+
+template <class M>
+struct MaskOps;
+template <>
+struct MaskOps<uint32_t> {
+    using V = uint2;  // visited word: (mask, stamp)
+    static constexpr int BITS = 32;
+    static __device__ __forceinline__ uint32_t get(const V& e, uint32_t st) { return e.y == st ? e.x : 0u; }
+    static __device__ __forceinline__ V make(uint32_t m, uint32_t st) { return make_uint2(m, st); }
+    static __device__ __forceinline__ uint32_t popc(uint32_t m) { return (uint32_t)__popc(m); }
+    static __device__ __forceinline__ uint32_t ctz(uint32_t m) { return (uint32_t)__builtin_ctz(m); }
+};
+template <>
+struct MaskOps<uint64_t> {
+    using V = uint4;  // visited word: (mask lo, mask hi, stamp, -)
+    static constexpr int BITS = 64;
+    static __device__ __forceinline__ uint64_t get(const V& e, uint32_t st) {
+        return e.z == st ? (((uint64_t)e.y << 32) | e.x) : 0ull;
+    }
+    static __device__ __forceinline__ V make(uint64_t m, uint32_t st) {
+        return make_uint4((uint32_t)m, (uint32_t)(m >> 32), st, 0u);
+    }
+    static __device__ __forceinline__ uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
+    static __device__ __forceinline__ uint32_t ctz(uint64_t m) { return (uint32_t)__builtin_ctzll(m); }
+};
+
+template <class M>
+struct RLds {
+    uint32_t hkey[H];                    // target + 1 (window-local); 0 = empty
+    M hmask[H];                          // OR of the masks folded into the target this round
+    uint16_t ent[H];                     // occupied hash slots in insertion order
+    unsigned long long s_fill[MaskOps<M>::BITS];  // counts-only: U fills per slot (A pass)
+    unsigned long long acc[6];           // workgroup totals: fills, te, reach, checks, levels, expansions
+    unsigned long long cte, creach;      // the current mini-group's TE and |R|
+    uint32_t qcnt[2];
+    uint32_t cursor;
+    uint32_t tcnt[2];                    // inserts of this round, by round parity
+    uint32_t held[2];                    // some group holds unfolded edges, by round parity
+    uint32_t chunk;
+};
+
+// Hash slot of window-local target m, inserting it if new; FULLI when it is
+// new and the round already holds T targets.  Linear probing.
+template <class M>
+__device__ __forceinline__ uint32_t rtab_find(RLds<M>& S, uint32_t* tcnt, uint32_t m) {
+    const uint32_t key = m + 1u;
+    uint32_t s = (m * 2654435761u) >> (32 - HB);
+    for (int probe = 0; probe < H; probe++) {
+        const uint32_t cur = S.hkey[s];
+        if (cur == key) return s;
+        if (cur == 0u) {
+            if (*tcnt >= (uint32_t)T) return FULLI;
+            const uint32_t prev = atomicCAS(&S.hkey[s], 0u, key);
+            if (prev == 0u) {
+                S.ent[atomicAdd(tcnt, 1u)] = (uint16_t)s;
+                return s;
+            }
+            if (prev == key) return s;
+        }
+        s = (s + 1u) & (uint32_t)(H - 1);
+    }
+    return FULLI;
+}
+
+// Slots j < count whose source first - j*stride lies above global vertex mg.
+template <class M>
+__device__ __forceinline__ M above_mask(uint32_t mg, uint32_t first, uint32_t stride, uint32_t count) {
+    if (mg >= first) return (M)0;
+    const uint32_t d = first - mg;
+    uint32_t c = stride == 1u ? d : (d + stride - 1u) / stride;
+    c = c < count ? c : count;
+    return c >= (uint32_t)MaskOps<M>::BITS ? ~(M)0 : (((M)1 << c) - (M)1);
+}
+
+// A lane group's bits of a wave ballot.
+__device__ __forceinline__ uint32_t grp_bits(uint64_t b, uint32_t grp) {
+    return (uint32_t)(b >> (G * grp)) & ((1u << G) - 1u);
+}
+
+struct RTally {
+    unsigned long long fills, te, reach, checks, expansions;
+};
+
+template <class M>
+__global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
+    using O = MaskOps<M>;
+    using VW = typename O::V;
+    __shared__ RLds<M> S;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t w = tid / WAVE;
+    const uint32_t grp = lane / G;
+    const uint32_t j = lane % G;
+    const uint32_t W = p.W;
+    VW* vis = reinterpret_cast<VW*>(p.vis) + (uint64_t)blockIdx.x * W;
+    M* nmask0 = reinterpret_cast<M*>(p.nmask) + (uint64_t)blockIdx.x * 2 * W;
+    uint4* q0 = p.q + (uint64_t)blockIdx.x * 2 * W;
+    uint32_t epoch = p.bstate[blockIdx.x];
+    RTally t{0, 0, 0, 0, 0};
+    unsigned long long levels = 0;
+    for (uint32_t i = tid; i < (uint32_t)H; i += RB) {
+        S.hkey[i] = 0u;
+        S.hmask[i] = (M)0;
+    }
+    if (tid < 6) S.acc[tid] = 0;
+    if (tid == 0) {
+        S.qcnt[0] = S.qcnt[1] = 0;
+        S.cursor = 0;
+        S.tcnt[0] = S.tcnt[1] = S.held[0] = S.held[1] = 0;
+    }
+    uint32_t rb = 0;  // round parity
+    for (;;) {
+        if (tid == 0) {
+            S.chunk = atomicAdd(&p.ctrl->src_next, 1u);
+            S.cte = S.creach = 0;
+        }
+        if (tid < (uint32_t)O::BITS) S.s_fill[tid] = 0;
+        __syncthreads();
+        const uint32_t ck = S.chunk;
+        if (ck >= p.n_chunks) break;
+        const BlkChunk C = p.chunks[ck];
+        const uint32_t wb = p.mat_base[C.mat];
+        if (epoch >= p.loops) {  // stamps exhausted: clear this workgroup's visited words
+            for (uint32_t i = tid; i < W; i += RB) vis[i] = O::make((M)0, 0u);
+            epoch = 0;
+            __syncthreads();  // cleared before the sources are stamped
+        }
+        epoch++;
+        const uint32_t stamp = epoch;
+        // level 0: the sources themselves, each visited by and queued for its own slot
+        if (tid < C.count) {
+            const uint32_t sg = C.first - tid * C.stride;
+            const uint32_t s = sg - wb;
+            const M bit = (M)1 << tid;
+            vis[s] = O::make(bit, stamp);
+            nmask0[s] = bit;
+            const uint32_t r0 = (uint32_t)p.rp[sg], r1 = (uint32_t)p.rp[sg + 1];
+            q0[tid] = make_uint4(s, r0, r1, 0u);
+            t.te += r1 - r0;
+        }
+        if (tid == 0) S.qcnt[0] = C.count;
+        __syncthreads();
+
+        uint32_t cur = 0;
+        for (uint32_t lev = 0;; lev++) {
+            const uint32_t size = S.qcnt[cur];
+            __syncthreads();  // every thread has the size before it can change
+            if (size == 0) break;
+            const uint32_t nxt = cur ^ 1u;
+            M* maskc = nmask0 + (uint64_t)cur * W;
+            M* maskn = nmask0 + (uint64_t)nxt * W;
+            const uint4* qc = q0 + (uint64_t)cur * W;
+            uint4* qn = q0 + (uint64_t)nxt * W;
+            // lane group state (group-uniform): the held vertex's mask, edge
+            // cursor and row end; pend = lanes of the current edge group not yet folded
+            bool held = false, done = false;
+            uint32_t pos = 0, end = 0, pend = 0;
+            M vm = (M)0;
+            for (;;) {  // rounds
+                // ---- A. combine
+                for (;;) {
+                    const bool want = !held && !done && S.tcnt[rb] < (uint32_t)T;
+                    const uint64_t wl = __ballot(want && j == 0u);
+                    uint32_t base = 0;
+                    if (lane == 0 && wl) base = atomicAdd(&S.cursor, (uint32_t)__popcll(wl));
+                    base = bcast0(base);
+                    if (want) {
+                        const uint32_t idx = base + (uint32_t)__popcll(wl & ((1ull << (G * grp)) - 1ull));
+                        if (idx >= size) {
+                            done = true;
+                        } else {
+                            const uint4 e = qc[idx];
+                            const M mk = maskc[e.x];
+                            pos = e.y;
+                            end = e.z;
+                            pend = (1u << G) - 1u;
+                            vm = mk;
+                            held = e.y < e.z;
+                            if (j == 0u) {
+                                maskc[e.x] = (M)0;  // consumed; refilled two levels later
+                                t.checks += (unsigned long long)(e.z - e.y) * O::popc(mk);
+                                t.expansions++;
+                            }
+                        }
+                    }
+                    if (!__ballot(held)) {
+                        // no group holds edges: pop again while the queue and the table allow
+                        if (__ballot(!done && S.tcnt[rb] < (uint32_t)T)) continue;
+                        break;
+                    }
+                    bool failed = false;
+                    if (held) {
+                        const uint32_t e = pos + j;
+                        if (((pend >> j) & 1u) && e < end) {
+                            const uint32_t m = (uint32_t)p.ci[e] - wb;
+                            const uint32_t hs = rtab_find<M>(S, &S.tcnt[rb], m);
+                            if (hs == FULLI) failed = true;
+                            else atomicOr(&S.hmask[hs], vm);
+                        }
+                    }
+                    const uint32_t fm = grp_bits(__ballot(failed), grp);
+                    if (held) {
+                        if (fm) {
+                            pend = fm;  // keep the unplaced edges of this group for the next round
+                        } else {
+                            pos += G;
+                            pend = (1u << G) - 1u;
+                            held = pos < end;
+                        }
+                    }
+                    if (__ballot(fm != 0u)) break;  // the table is full for everyone
+                }
+                if (held && j == 0u) atomicOr(&S.held[rb], 1u);
+                __syncthreads();
+                // final for this round: the cursor moves only in phase A
+                const bool more = S.held[rb] != 0u || S.cursor < size;
+                // ---- B. relax: one lane per table entry
+                const uint32_t tc = S.tcnt[rb];
+                const uint32_t ntab = tc < (uint32_t)H ? tc : (uint32_t)H;
+                for (uint32_t b0 = w * WAVE; b0 < ntab; b0 += RB) {
+                    const uint32_t i = b0 + lane;
+                    const bool ok = i < ntab;
+                    uint32_t m = 0;
+                    M mk = (M)0;
+                    if (ok) {
+                        const uint32_t hs = S.ent[i];
+                        m = S.hkey[hs] - 1u;
+                        mk = S.hmask[hs];
+                        S.hkey[hs] = 0u;
+                        S.hmask[hs] = (M)0;
+                    }
+                    const uint32_t mg = m + wb;
+                    M nw = (M)0, bl = (M)0, up = (M)0;
+                    uint32_t r0 = 0, r1 = 0;
+                    bool enq = false;
+                    if (ok) {
+                        const VW vw = vis[m];
+                        const M mn = maskn[m];
+                        r0 = (uint32_t)p.rp[mg];
+                        r1 = (uint32_t)p.rp[mg + 1];
+                        const M vv = O::get(vw, stamp);
+                        nw = mk & ~vv;
+                        if (nw) {
+                            vis[m] = O::make(vv | nw, stamp);
+                            bl = nw & above_mask<M>(mg, C.first, C.stride, C.count);
+                            up = lev ? (nw & ~bl) : (M)0;  // level 0: A's own entries
+                            if (bl) {
+                                maskn[m] = mn | bl;
+                                enq = mn == (M)0;
+                                const uint32_t nb = O::popc(bl);
+                                t.reach += nb;
+                                t.te += (unsigned long long)nb * (r1 - r0);
+                            }
+                        }
+                    }
+                    const uint64_t em = __ballot(enq);
+                    if (em) {
+                        uint32_t qb = 0;
+                        if (lane == 0) qb = atomicAdd(&S.qcnt[nxt], (uint32_t)__popcll(em));
+                        qb = bcast0(qb);
+                        if (enq) qn[qb + rank_in(em)] = make_uint4(m, r0, r1, 0u);
+                    }
+                    const uint32_t nf = O::popc(up);
+                    t.fills += nf;
+                    if (__ballot(nf != 0u)) {
+                        if (p.fill_cap != 0ull) {
+                            // (row << 32) | column, one reservation per wave
+                            const uint32_t incl = wave_incl_scan(nf);
+                            const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+                            uint64_t fp = 0;
+                            if (lane == 0) fp = aadd64(&p.ctrl->fill_pos, total);
+                            fp = bcast0_64(fp) + (incl - nf);
+                            for (M b = up; b; b &= b - (M)1, fp++) {
+                                const uint32_t src = C.first - O::ctz(b) * C.stride;
+                                const uint64_t pr = p.transposed ? (((uint64_t)mg << 32) | src)
+                                                                 : (((uint64_t)src << 32) | mg);
+                                if (fp < p.fill_cap) p.fills[fp] = pr;
+                            }
+                        } else if (nf) {  // counts only: fills per row
+                            if (p.transposed) {
+                                aadd64(&p.row_fills[mg], nf);
+                            } else {
+                                for (M b = up; b; b &= b - (M)1) atomicAdd(&S.s_fill[O::ctz(b)], 1ull);
+                            }
+                        }
+                    }
+                }
+                // the next round's counters, cleared while others still relax
+                // (the other parity was last read before the previous barrier)
+                if (tid == 0) {
+                    S.tcnt[rb ^ 1u] = 0;
+                    S.held[rb ^ 1u] = 0;
+                }
+                __syncthreads();
+                rb ^= 1u;
+                if (!more) break;
+            }
+            if (tid == 0) {
+                S.qcnt[cur] = 0;
+                S.cursor = 0;
+            }
+            __syncthreads();
+            cur = nxt;
+            levels++;
+        }
+        // the mini-group's TE and |R| (per matrix); counts-only fills per source row
+        const unsigned long long te = wave_sum64(t.te), rc = wave_sum64(t.reach);
+        if (lane == 0 && (te | rc)) {
+            atomicAdd(&S.cte, te);
+            atomicAdd(&S.creach, rc);
+        }
+        t.te = t.reach = 0;
+        __syncthreads();
+        if (tid == 0) {
+            atomicAdd(&S.acc[1], S.cte);
+            atomicAdd(&S.acc[2], S.creach);
+            if (!p.transposed && p.mat_te) {
+                aadd64(&p.mat_te[C.mat], S.cte);
+                aadd64(&p.mat_reach[C.mat], S.creach);
+            }
+        }
+        if (p.fill_cap == 0ull && !p.transposed && tid < C.count)
+            p.row_fills[C.first - tid * C.stride] = S.s_fill[tid];
+        __syncthreads();
+    }
+    if (t.fills) atomicAdd(&S.acc[0], t.fills);
+    if (t.checks) atomicAdd(&S.acc[3], t.checks);
+    if (t.expansions) atomicAdd(&S.acc[5], t.expansions);
+    __syncthreads();
+    if (tid == 0) {
+        Ctrl* c = p.ctrl;
+        if (S.acc[0]) aadd64(&c->fill_count, S.acc[0]);
+        if (p.transposed) {
+            if (S.acc[1]) aadd64(&c->te_t, S.acc[1]);
+            if (S.acc[2]) aadd64(&c->reach_t, S.acc[2]);
+        } else {
+            if (S.acc[1]) aadd64(&c->te, S.acc[1]);
+            if (S.acc[2]) aadd64(&c->reach, S.acc[2]);
+        }
+        if (S.acc[3]) aadd64(&c->edge_checks, S.acc[3]);
+        if (S.acc[5]) aadd64(&c->expansions, S.acc[5]);
+        aadd64(&c->levels, levels);
+        p.bstate[blockIdx.x] = epoch;
+    }
+}
+
+}  // namespace
+
+int rch_block_size() { return RB; }
+bool rch_bits_valid(int bits) { return bits == 32 || bits == 64; }
+const void* rch_kernel_ptr(int bits) {
+    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<uint64_t>)
+                      : reinterpret_cast<const void*>(&k_reach_blk<uint32_t>);
+}
+// Per workgroup: visited words (8 or 16 B), two next-level masks, two
+// 16-byte frontier queues, per vertex of the window.
+int64_t reach_bytes(uint32_t nb, uint32_t W, int bits) {
+    const int64_t vw = bits == 64 ? 16 : 8, mw = bits / 8;
+    return (int64_t)nb * ((vw + 2 * mw + 32) * (int64_t)W + 8);
+}
+
+hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s) {
+    if (bits == 64) hipLaunchKernelGGL(k_reach_blk<uint64_t>, dim3(p.nblocks), dim3(RB), 0, s, p);
+    else if (bits == 32) hipLaunchKernelGGL(k_reach_blk<uint32_t>, dim3(p.nblocks), dim3(RB), 0, s, p);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}  // namespace gsofa

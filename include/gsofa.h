@@ -91,7 +91,8 @@ typedef struct gsofa_options {
                                   /* (rounded up to whole groups) first run as grid-mode    */
                                   /* groups (GSOFA_MODE_HYBRID); 0 = none                   */
     int32_t block_slots;          /* block mode: sources per workgroup, 32, 16 or 8; 0 = 32 */
-                                  /* (fewer slots, larger per-round target table)            */
+                                  /* (fewer slots, larger per-round target table).  Reach   */
+                                  /* mode: 32 or 64 (the bits of a visited word); 0 = auto   */
 } gsofa_options;
 
 /* Execution modes of the traversal (identical results):
@@ -104,11 +105,18 @@ typedef struct gsofa_options {
  *        symbfact.cuh:706-962).
  * HYBRID (reported, chosen with top_sources): the deepest sources, whose
  *        traversals are long chains of wide levels, run as GRID groups that
- *        spread every level over the whole device; the rest run in BLOCK mode. */
+ *        spread every level over the whole device; the rest run in BLOCK mode.
+ * REACH: two label-free reachability passes with a visited bit per source
+ *        slot: over A (U rows, TE_alg, |R|) and over A^T (L rows, since
+ *        struct(L) of A is struct(U) of A^T transposed).  Every vertex is
+ *        expanded at most once per source.  Whole-matrix runs only (one share
+ *        or every share in one execute, in core); per-row TE and |R| are not
+ *        reported (row_te = row_reach = NULL), their totals are. */
 #define GSOFA_MODE_AUTO 0
 #define GSOFA_MODE_BLOCK 1
 #define GSOFA_MODE_GRID 2
 #define GSOFA_MODE_HYBRID 3
+#define GSOFA_MODE_REACH 4
 
 void gsofa_options_default(gsofa_options* opt);
 

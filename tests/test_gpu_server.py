@@ -133,15 +133,17 @@ def test_server_runs_oversized_alone():
 
 def test_server_spills_what_does_not_fit_in_core():
     """A request whose in-core output does not fit the whole budget (a
-    100K-row band reserves about 0.95 GB for its fill stream and columns,
-    against 0.5 GiB) runs alone in spilled shares and equals its in-core
+    100K-row band of half-width 64 reserves about 0.17 GiB for its per-row
+    arrays, fill stream and L/U columns, sized from its exact row bounds,
+    against 0.15 GiB) runs alone in spilled shares and equals its in-core
     stand-alone run."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     small = cf.batch_matrix(20, 1)
     big = g.gen_band(100000, 64, 8.0, 3)
-    _say("case: server budget 0.5 GiB, one request served in spilled shares")
-    with g.Server(max_inflight=1, max_batch=4, mem_gb=0.5) as srv:
+    assert g.footprint(big) > int(0.15 * 2**30)
+    _say("case: server budget 0.15 GiB, one request served in spilled shares")
+    with g.Server(max_inflight=1, max_batch=4, mem_gb=0.15) as srv:
         ts = [srv.submit(small), srv.submit(big), srv.submit(small)]
         out = [srv.wait(t) for t in ts]
     assert out[1][1]["spill_shares"] > 1 and out[1][1]["batch_size"] == 1
