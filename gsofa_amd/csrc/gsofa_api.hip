@@ -610,6 +610,18 @@ int block_grid(const gsofa_options& o, int S, uint64_t nchunks, uint32_t W, int6
     return GSOFA_OK;
 }
 
+// The mode AUTO resolves to: reach mode wherever it applies (one share, in
+// core, no hybrid split, sources per workgroup left open), else block mode
+// (grid mode for block-mode matrices past 32-bit column offsets is decided at
+// plan creation).
+int resolve_mode(const gsofa_options& o) {
+    if (o.mode != GSOFA_MODE_AUTO) return o.mode;
+    const bool one_share = o.n_virtual_gpu <= 1;
+    if (one_share && !(o.flags & GSOFA_FLAG_SPILL) && o.top_sources <= 0 && o.block_slots == 0)
+        return GSOFA_MODE_REACH;
+    return GSOFA_MODE_AUTO;
+}
+
 // Reach mode: sources per workgroup (32 or 64).  Unless the options fix it,
 // a single matrix takes 64 when its chunks still give every co-resident
 // workgroup one, else 32; batches take 64.
@@ -707,6 +719,7 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     if (opt) o = *opt;
     else gsofa_options_default(&o);
     if (!A || A->n <= 0) return 0;
+    o.mode = resolve_mode(o);
     if (o.mode == GSOFA_MODE_GRID || (o.mode == GSOFA_MODE_AUTO && A->nnz >= kMaxBlockNnz)) {
         Sizing sz;
         if (choose_sizing(A, &o, A->nnz, INT64_MAX / 4, &sz) != GSOFA_OK) return -1;
@@ -771,12 +784,13 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     P->h_mat_base.assign(mat_base, mat_base + n_mats + 1);
     P->W = 1;
     for (int32_t b = 0; b < n_mats; b++) P->W = std::max<uint32_t>(P->W, mat_base[b + 1] - mat_base[b]);
-    if (n_mats > 1) {  // batches: one share, block (or reach) mode
+    if (n_mats > 1) {  // batches: one share, reach or block mode
         P->opt.n_virtual_gpu = 1;
         P->opt.virtual_gpu_id = -1;
-        if (P->opt.mode != GSOFA_MODE_REACH) P->opt.mode = GSOFA_MODE_BLOCK;
+        if (P->opt.mode != GSOFA_MODE_REACH && P->opt.mode != GSOFA_MODE_AUTO) P->opt.mode = GSOFA_MODE_BLOCK;
     }
     if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
+    P->opt.mode = resolve_mode(P->opt);
     const bool reach = P->opt.mode == GSOFA_MODE_REACH;
     if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu ||
         (P->opt.block_slots != 0 && !(reach ? rch_bits_valid(P->opt.block_slots)

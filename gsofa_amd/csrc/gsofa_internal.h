@@ -135,6 +135,7 @@ const void* rch_kernel_ptr(int bits);
 int64_t reach_bytes(uint32_t nb, uint32_t W, int bits);
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s);
 int reach_resident(int device, uint32_t* out);
+int resolve_mode(const gsofa_options& o);
 int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits);
 int reach_grid(const gsofa_options& o, int bits, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
 void transpose_rows(int32_t n, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& rpT,

@@ -81,27 +81,32 @@ struct gsofa_server {
 namespace {
 
 constexpr int32_t kMaxSpill = 256;  // most spilled shares tried for one request
+constexpr int kReachBits = 64;      // sources per workgroup of an in-core batch (reach mode)
 
 // Bytes the batch `reqs` would need apart from its traversal arrays: CSR,
 // per-row arrays and the output reserved from the requests' bounds (as
-// gsofa_plan_create reserves it).  With `shares` spilled shares (one request)
-// the device holds one share's output: the cyclic shares' bounds are close to
-// equal, and a quarter more covers the largest.
+// gsofa_plan_create reserves it).  In core a batch runs in reach mode (64
+// sources per workgroup), which also holds A^T.  With `shares` spilled shares
+// (one request, block mode) the device holds one share's output: the cyclic
+// shares' bounds are close to equal, and a quarter more covers the largest.
 int64_t fixed_bytes(const std::vector<Request*>& reqs, int32_t shares = 1) {
     int64_t n = 0, nnz = 0;
     uint64_t nchunks = 0, fb = 0, lu = 0;
+    const uint64_t per = shares > 1 ? (uint64_t)gsofa::kBatchSlots : (uint64_t)kReachBits;
     for (const Request* r : reqs) {
         n += r->n;
         nnz += r->nnz;
         fb += r->fill_bound;
         lu += r->lu_bound;
-        nchunks += ((uint64_t)r->n + gsofa::kBatchSlots - 1) / gsofa::kBatchSlots;
+        nchunks += 2 * (((uint64_t)r->n + per - 1) / per);
     }
     if (shares > 1) {
         fb = fb / (uint64_t)shares + fb / (4 * (uint64_t)shares) + 1;
         lu = lu / (uint64_t)shares + lu / (4 * (uint64_t)shares) + 1;
     }
-    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, std::max<uint64_t>(fb, 1), (int64_t)lu);
+    const int64_t transposed = shares > 1 ? 0 : 4 * (n + 1) + 4 * nnz + 16 * (int64_t)reqs.size();
+    return gsofa::block_fixed_bytes(n, nnz, (int32_t)reqs.size(), nchunks, std::max<uint64_t>(fb, 1), (int64_t)lu) +
+           transposed;
 }
 
 uint32_t largest(const std::vector<Request*>& reqs) {
@@ -110,9 +115,15 @@ uint32_t largest(const std::vector<Request*>& reqs) {
     return W;
 }
 
+// Traversal bytes of one workgroup: reach mode in core, block mode spilled.
+int64_t per_workgroup(const std::vector<Request*>& reqs, int32_t shares = 1) {
+    return shares > 1 ? gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots)
+                      : gsofa::reach_bytes(1, largest(reqs), kReachBits);
+}
+
 // Traversal bytes left for the batch within `budget` (< 0: does not fit).
 int64_t trav_room(int64_t budget, const std::vector<Request*>& reqs, int32_t shares = 1) {
-    return budget - fixed_bytes(reqs, shares) - gsofa::block_bytes(1, largest(reqs), gsofa::kBatchSlots);
+    return budget - fixed_bytes(reqs, shares) - per_workgroup(reqs, shares);
 }
 
 void finish(gsofa_server* S, Request* r, int status) {
@@ -133,10 +144,14 @@ void run_batch(gsofa_server* S, std::vector<Request*>& batch, int32_t batch_id, 
     o.device = S->opt.device;
     o.max_supernode_size = S->opt.max_supernode_size;
     o.flags = S->opt.flags;
-    o.block_slots = gsofa::kBatchSlots;  // also for a batch of one: the sizing above assumes it
-    if (spill > 1) {  // a single request, alone on the device
+    if (spill > 1) {  // a single request, alone on the device: block mode, spilled shares
+        o.mode = GSOFA_MODE_BLOCK;
+        o.block_slots = gsofa::kBatchSlots;
         o.flags |= GSOFA_FLAG_SPILL;
         o.n_virtual_gpu = spill;
+    } else {  // in core: reach mode (the sizing above assumes 64 sources per workgroup)
+        o.mode = GSOFA_MODE_REACH;
+        o.block_slots = kReachBits;
     }
     // the traversal arrays get what the fixed part leaves of the budget
     o.mem_budget_bytes = std::max<int64_t>(budget - fixed_bytes(batch, spill), 1);
@@ -199,15 +214,22 @@ void worker_main(gsofa_server* S) {
             S->waiting.pop_front();
             const std::vector<Request*> one{r};
             const int32_t most = std::min<int32_t>(kMaxSpill, std::max<int32_t>(r->n, 1));
-            const int64_t per_wg = gsofa::block_bytes(1, largest(one), gsofa::kBatchSlots);
             auto wgs = [&](int32_t sh) {  // traversal workgroups that fit with sh shares
+                const int64_t per_wg = per_workgroup(one, sh);
                 const int64_t room = S->opt.mem_budget_bytes - fixed_bytes(one, sh);
                 return room < per_wg ? (int64_t)0 : std::min<int64_t>(room / per_wg, (int64_t)S->resident);
             };
+            // in core (reach mode) when the whole budget holds it with at
+            // least one workgroup; otherwise the fewest spilled shares
+            // (block mode) that keep 7/8 of the workgroups the most shares would
             int32_t top = 1;
             while (top * 2 <= most) top *= 2;
-            const int64_t best = wgs(top);
-            while (spill < top && 8 * wgs(spill) < 7 * best) spill *= 2;
+            const int64_t best = std::max(wgs(1), top > 1 ? wgs(top) : (int64_t)0);
+            if (wgs(1) == 0) {
+                spill = 2;
+                while (spill < top && 8 * wgs(spill) < 7 * wgs(top)) spill *= 2;
+                if (spill > top) spill = top;
+            }
             if (best == 0) {
                 r->t_start = Clock::now();
                 finish(S, r, GSOFA_ERR_BUDGET);
@@ -292,8 +314,7 @@ int gsofa_server_create(const gsofa_server_options* opt, gsofa_server** out) {
         o.mem_budget_bytes = (int64_t)((double)freeb * 0.80);
     }
     uint32_t resident = 0;
-    if (hipSetDevice(o.device) != hipSuccess ||
-        gsofa::resident_blocks(gsofa::kBatchSlots, o.device, &resident) != GSOFA_OK) {
+    if (hipSetDevice(o.device) != hipSuccess || gsofa::reach_resident(o.device, &resident) != GSOFA_OK) {
         (void)hipGetLastError();
         return GSOFA_ERR_HIP;
     }

@@ -52,6 +52,26 @@ def test_reach_bcsstk18(bcsstk18, bits):
     assert_same_structure(r, oracle_of(bcsstk18))
 
 
+def test_reach_is_automatic(bcsstk18):
+    """AUTO takes reach mode for whole-matrix runs in core: 64 sources per
+    workgroup when the chunks still give every co-resident workgroup one,
+    else 32; block mode where it is asked for by slots, shares or spill."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    _say("case: automatic mode")
+    r = g.symbfact(bcsstk18, counts_only=True)
+    assert r.mode == "reach" and r.chunk // r.grid_blocks == 32 and r.fill_count == 980136
+    B = cf.band(200000)
+    r = g.symbfact(B, counts_only=True)
+    assert r.mode == "reach" and r.chunk // r.grid_blocks == 64
+    assert g.symbfact(bcsstk18, counts_only=True, block_slots=16).mode == "block"
+    assert g.symbfact(bcsstk18, counts_only=True, n_virtual_gpu=2).mode == "block"
+    with g.Plan(cf.grid2d_nd_batch()[:8]) as p:
+        p.execute()
+        r = p.fetch_matrix(3)
+        assert r.mode == "reach" and r.chunk // r.grid_blocks == 64
+
+
 @pytest.mark.parametrize("bits", BITS)
 def test_reach_grid64(bits):
     import gsofa_amd as g

@@ -113,15 +113,15 @@ def test_server_budget_rejects_oversized():
 
 def test_server_runs_oversized_alone():
     """A request that does not fit a worker's share but fits the whole budget
-    (a 200K-row band: about 1.9 GB, mostly its reserved fill stream, against
-    1.5 GiB shares of 3 GiB) waits for the device to drain and runs alone;
-    every result equals its stand-alone run."""
+    (a 200K-row band: about 1.4 GB, mostly its fill stream and L/U columns
+    sized from its row bounds, against 1 GiB shares of 2 GiB) waits for the
+    device to drain and runs alone; every result equals its stand-alone run."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     small = cf.batch_matrix(20, 1)
     big = g.gen_band(200000, 256, 8.0, 3)
-    _say("case: server budget 3 GiB in 2 shares, one request above a share")
-    with g.Server(max_inflight=2, max_batch=4, mem_gb=3.0) as srv:
+    _say("case: server budget 2 GiB in 2 shares, one request above a share")
+    with g.Server(max_inflight=2, max_batch=4, mem_gb=2.0) as srv:
         ts = [srv.submit(small), srv.submit(big), srv.submit(small), srv.submit(small)]
         out = [srv.wait(t) for t in ts]
     assert out[1][1]["batch_size"] == 1
@@ -219,7 +219,8 @@ def test_server_two_waiters_on_one_ticket():
 
 def test_server_counts_only_matches_oracle():
     """A counts-only server returns each request's fill count, per-row fills,
-    TE and |R| (no L/U), equal to the oracle's."""
+    TE and |R| (no L/U), equal to the oracle's (per-row TE and |R| where the
+    mode reports them: in reach mode their totals)."""
     import gsofa_amd as g
     from oracle import oracle
     reqs = _requests()[:20]
@@ -231,6 +232,7 @@ def test_server_counts_only_matches_oracle():
         o = oracle.symbfact(A.colbeg, A.colend, A.csr, A.n)
         assert r.fill_count == o.fill_count and r.te_alg == o.te_alg and r.reach == o.reach
         np.testing.assert_array_equal(r.row_fill, o.row_fill)
-        np.testing.assert_array_equal(r.row_te, o.row_te)
-        np.testing.assert_array_equal(r.row_reach, o.row_reach)
+        if r.row_te is not None:
+            np.testing.assert_array_equal(r.row_te, o.row_te)
+            np.testing.assert_array_equal(r.row_reach, o.row_reach)
         assert r.L_rowptr is None and r.n_super == -1

@@ -135,19 +135,20 @@ def test_slots_grid3d_nd_full_golden(slots):
 
 
 def test_auto_slots(bcsstk18):
-    """With block_slots left at 0: the largest slot count whose chunks still
-    cover every co-resident workgroup for a single matrix, 16 for batches."""
+    """Block mode with block_slots left at 0: the largest slot count whose
+    chunks still cover every co-resident workgroup for a single matrix, 16 for
+    batches."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     _say("case: automatic slots")
-    r = g.symbfact(bcsstk18, counts_only=True)
+    r = g.symbfact(bcsstk18, counts_only=True, mode="block")
     assert r.chunk // r.grid_blocks == 32 and r.fill_count == 980136
     A = g.gen_band(300, 8, 3.0, 1)
-    r = g.symbfact(A, counts_only=True)
+    r = g.symbfact(A, counts_only=True, mode="block")
     assert r.chunk // r.grid_blocks == 8
     assert r.fill_count == oracle_of(A).fill_count
     mats = cf.grid2d_nd_batch()[:16]
-    with g.Plan(mats, counts_only=True) as p:
+    with g.Plan(mats, counts_only=True, mode="block") as p:
         p.execute()
         r = p.fetch_matrix(0)
         assert r.chunk // r.grid_blocks == 16
