@@ -896,7 +896,20 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         P->S = (uint32_t)bits;
         P->rbits = bits;
         if (!rc) {
+            // the A pass's chunks, deepest first, each followed by the same
+            // sources' chunk of the A^T pass: one launch runs both
             build_chunks(P);
+            std::vector<BlkChunk> both;
+            both.reserve(2 * P->h_chunks.size());
+            for (const BlkChunk& ch : P->h_chunks) {
+                both.push_back(ch);
+                BlkChunk t = ch;
+                t.stride |= rch_pass_bit();
+                both.push_back(t);
+            }
+            P->h_chunks.swap(both);
+            P->share_chunk_off.assign(2, 0);
+            P->share_chunk_off[1] = (uint32_t)P->h_chunks.size();
             transpose_rows(n, P->h_rp.data(), ci.data(), rpT, ciT);
             const int64_t tbytes = 4 * (int64_t)(n + 1) + 4 * P->nnz + 16 * (int64_t)n_mats;
             rc = reach_grid(P->opt, bits, std::max<uint64_t>(P->h_chunks.size(), 1), P->W,
@@ -1185,6 +1198,8 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         RchParams rp{};
         rp.rp = P->d_rp;
         rp.ci = P->d_ci;
+        rp.rpT = P->d_rpT;
+        rp.ciT = P->d_ciT;
         rp.W = P->W;
         rp.chunks = P->d_chunks;
         rp.n_chunks = (uint32_t)P->h_chunks.size();
@@ -1203,17 +1218,11 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         rp.loops = std::min(0xFFFFFFFFu, P->span_cap);
         HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_mat_te), 2 * (uint64_t)P->n_mats, 0u, P->stream));
         HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_mat_reach), 2 * (uint64_t)P->n_mats, 0u, P->stream));
+        if (!P->structure)  // both passes add their fills per row
+            HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_row_fk), 2 * (uint64_t)n, 0u, P->stream));
         HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
         HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
-        if (rp.n_chunks > 0) {
-            // over A: U rows, TE, |R|; then over A^T: L rows
-            HIPCHK(launch_reach(rp, P->rbits, P->stream));
-            HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
-            rp.rp = P->d_rpT;
-            rp.ci = P->d_ciT;
-            rp.transposed = 1;
-            HIPCHK(launch_reach(rp, P->rbits, P->stream));
-        }
+        if (rp.n_chunks > 0) HIPCHK(launch_reach(rp, P->rbits, P->stream));
         HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
         return GSOFA_OK;
     }

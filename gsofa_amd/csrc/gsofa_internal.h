@@ -106,11 +106,14 @@ int resident_blocks(int slots, int device, uint32_t* out);
 const void* blk_kernel_ptr(int slots);
 hipError_t launch_traverse_blk(const BlkParams& p, int slots, size_t lds_bytes, hipStream_t s);
 
-// Reach mode (gsofa_reach.hip): the same work table, run once over A (U rows,
-// TE, |R|) and once over A^T (L columns, written as (row, column) fills).
+// Reach mode (gsofa_reach.hip): one launch over a work table whose chunks
+// run over A (U rows, TE, |R|) or, with rch_pass_bit() set in their stride,
+// over A^T (L columns, written as (row, column) fills).
 struct RchParams {
-    const int32_t* rp;     // packed row pointers of A, or of A^T in the transposed pass
+    const int32_t* rp;     // packed row pointers and columns of A
     const int32_t* ci;
+    const int32_t* rpT;    // of A^T
+    const int32_t* ciT;
     uint32_t W;            // per-workgroup window: largest matrix size
     const BlkChunk* chunks;
     uint32_t n_chunks;
@@ -123,16 +126,16 @@ struct RchParams {
     uint64_t fill_cap;     // 0: count fills per row without writing the fill stream
     Ctrl* ctrl;
     uint32_t* bstate;      // per workgroup: stamp of its last mini-group
-    unsigned long long* row_fills;  // counts only: fills per row (the A pass stores, A^T adds)
-    unsigned long long* mat_te;     // per matrix TE and |R| (A pass), or null
+    unsigned long long* row_fills;  // counts only: fills per row, added by both passes (zeroed first)
+    unsigned long long* mat_te;     // per matrix TE and |R| of the A pass (zeroed first), or null
     unsigned long long* mat_reach;
     uint32_t loops;        // stamps per workgroup before its visited words are cleared
-    uint32_t transposed;   // 1: the A^T pass
 };
 int rch_block_size();
 bool rch_bits_valid(int bits);
 const void* rch_kernel_ptr(int bits);
 int64_t reach_bytes(uint32_t nb, uint32_t W, int bits);
+uint32_t rch_pass_bit();
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s);
 int reach_resident(int device, uint32_t* out);
 int resolve_mode(const gsofa_options& o);

@@ -7,35 +7,41 @@
 //   L_s = {m < s : s is reached from m through vertices below m in A^T}, since
 //         a path s -> ... -> m of A whose intermediates lie below m is a path
 //         m -> ... -> s of A^T whose intermediates lie below its source m.
-// So one kernel answers both: run over A it yields U rows, |R| and TE exactly
-// as the reference's traversal does for U (with_space_optimization/
-// symbfact.cuh:883-1082, the "m > src" branch at :992-1009); run over A^T it
+// So one kernel answers both: over A it yields U rows, |R| and TE exactly as
+// the reference's traversal does for U (with_space_optimization/
+// symbfact.cuh:883-1082, the "m > src" branch at :992-1009); over A^T it
 // yields L columns, written to the fill stream as (row, column) pairs so the
 // structure pass builds the same sorted rows.  Each (vertex, source) pair is
 // expanded at most once, where the reference's max-id labels re-expand a
-// vertex each time its label drops (5-9 times per pair on 3D nested-dissection
-// grids).
+// vertex each time its label drops (2-10 times per pair on 3D
+// nested-dissection grids).
 //
 // Execution model (CDNA4, one 1024-thread workgroup per CU, persistent grid):
 // * A workgroup owns MB source slots (32 or 64: the bits of a mask word) and
-//   runs a mini-group of up to MB sources of one matrix through all of their
-//   levels alone; the next mini-group comes from one global counter.
+//   runs a mini-group of up to MB sources of one matrix, in one of the two
+//   passes, through all of its levels alone.  The next mini-group comes from
+//   one global counter over a work table that interleaves the passes' chunks,
+//   deepest first, so both passes share one launch and its tail.
 // * Per vertex, one word holds the MB slots' visited bits with the
 //   mini-group's stamp beside it; an older stamp reads as "nothing visited",
 //   so the array is never cleared between mini-groups (the reference's
 //   source-stamped markers, symbfact.cuh:2126, generalised to a bit per slot).
 // * A level runs in rounds of two phases separated by workgroup barriers:
-//     A. combine: lane groups of G lanes pop frontier vertices; lane j of a
-//        group takes edge j of the current group of G edges, finds or inserts
-//        the target in an LDS hash table of the round's distinct targets and
-//        ORs the vertex's slot mask into the target's mask (ds_or);
+//     A. combine: lane groups of G = 8 lanes pop frontier vertices (all
+//        lanes of a group load the same queue entry and mask, so nothing is
+//        broadcast between lanes); lane j of a group takes edge j of the current
+//        group of G edges, finds or inserts the target in an LDS hash table
+//        of the round's distinct targets and ORs the vertex's slot mask into
+//        the target's mask (ds_or);
 //     B. relax: one lane per table entry: new = mask & ~visited; the new bits
 //        of slots whose source lies above the target push it to the next
 //        frontier (its R_<(s) membership), the others are U entries (fills
 //        from level 1 on: level 0 expands the sources themselves, whose
 //        neighbours are A's own entries).
 //   Targets are unique within a round, so the visited word, the next-level
-//   mask and the queue append need no global atomics.
+//   mask and the queue append need no global atomics.  The round's counters
+//   alternate by parity and the level's by level parity, so a round costs two
+//   barriers and a level no more.
 // * Sources of a mini-group form an arithmetic sequence first, first-stride,
 //   ... (the cyclic source map of symbfact.cuh:1084-1099 within a share), so
 //   "source j lies above m" is the prefix of slots j < ceil((first-m)/stride).
@@ -48,12 +54,15 @@ namespace {
 
 using namespace dev;
 
+#ifndef GSOFA_RHB
+#define GSOFA_RHB 13
+#endif
 constexpr int RB = 1024;              // threads per workgroup
-constexpr int G = 8;                  // lanes per frontier vertex in combine
-constexpr int HB = 13;                // log2 hash slots
+constexpr int HB = GSOFA_RHB;         // log2 hash slots
 constexpr int H = 1 << HB;
 constexpr int T = H / 2;              // inserts allowed per round (load <= 1/2 + RB/H)
 constexpr uint32_t FULLI = 0xFFFFFFFFu;
+constexpr uint32_t PASS_T = 0x80000000u;  // chunk stride bit: the A^T pass
 static_assert(T + RB <= H, "insert overshoot stays within the entry list");
 
 template <class M>
@@ -86,11 +95,12 @@ struct RLds {
     uint32_t hkey[H];                    // target + 1 (window-local); 0 = empty
     M hmask[H];                          // OR of the masks folded into the target this round
     uint16_t ent[H];                     // occupied hash slots in insertion order
-    unsigned long long s_fill[MaskOps<M>::BITS];  // counts-only: U fills per slot (A pass)
-    unsigned long long acc[6];           // workgroup totals: fills, te, reach, checks, levels, expansions
+    unsigned long long s_fill[MaskOps<M>::BITS];  // counts only: U fills per slot (A pass)
+    unsigned long long acc[8];           // workgroup totals: fills, checks, -, expansions,
+                                         // TE and |R| of the A pass, of the A^T pass
     unsigned long long cte, creach;      // the current mini-group's TE and |R|
-    uint32_t qcnt[2];
-    uint32_t cursor;
+    uint32_t qcnt[2];                    // queue sizes, by buffer
+    uint32_t cursor[2];                  // pop cursors, by level parity
     uint32_t tcnt[2];                    // inserts of this round, by round parity
     uint32_t held[2];                    // some group holds unfolded edges, by round parity
     uint32_t chunk;
@@ -102,6 +112,10 @@ template <class M>
 __device__ __forceinline__ uint32_t rtab_find(RLds<M>& S, uint32_t* tcnt, uint32_t m) {
     const uint32_t key = m + 1u;
     uint32_t s = (m * 2654435761u) >> (32 - HB);
+    {
+        const uint32_t cur = S.hkey[s];  // usual case: present at its first slot
+        if (cur == key) return s;
+    }
     for (int probe = 0; probe < H; probe++) {
         const uint32_t cur = S.hkey[s];
         if (cur == key) return s;
@@ -129,16 +143,103 @@ __device__ __forceinline__ M above_mask(uint32_t mg, uint32_t first, uint32_t st
     return c >= (uint32_t)MaskOps<M>::BITS ? ~(M)0 : (((M)1 << c) - (M)1);
 }
 
-// A lane group's bits of a wave ballot.
-__device__ __forceinline__ uint32_t grp_bits(uint64_t b, uint32_t grp) {
-    return (uint32_t)(b >> (G * grp)) & ((1u << G) - 1u);
+// The calling lane group's bits of a wave ballot (bit k: lane k of the group).
+template <int G>
+__device__ __forceinline__ uint32_t grp_ballot(bool x, uint32_t grp) {
+    return (uint32_t)(__ballot(x) >> (G * grp)) & ((1u << G) - 1u);
 }
 
-struct RTally {
-    unsigned long long fills, te, reach, checks, expansions;
+// The frontier vertex a lane group holds across rounds (group-uniform: every
+// lane of the group loads the same queue entry and mask, so no broadcast is
+// needed): its slot mask, edge cursor, row end and the lanes of its current
+// edge group not yet folded.
+template <class M>
+struct RGrp {
+    bool held;           // a vertex is in progress
+    bool done;           // the queue is exhausted for this level
+    uint32_t pos, end, pend;
+    M vm;
 };
 
-template <class M>
+struct RTally {
+    unsigned long long fills, checks, expansions, te, reach;
+};
+
+// The mini-group being run: its chunk, matrix window and pass.
+struct RCur {
+    const int32_t* rp;
+    const int32_t* ci;
+    uint32_t first, count, stride, mat, wb;
+    bool tpass;
+};
+
+// Phase A: lane groups pop frontier vertices one at a time and fold each
+// edge group's targets into the table until the queue is exhausted or the
+// table is full.  Wave-uniform control flow; the groups hold independent
+// vertices.
+template <int G, class M>
+__device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const uint4* qc, uint32_t size,
+                         uint32_t* cursor, uint32_t rb, RTally& t) {
+    using O = MaskOps<M>;
+    const uint32_t lane = __lane_id();
+    const uint32_t grp = lane / G;
+    const uint32_t j = lane % G;
+    constexpr uint32_t ALL = (1u << G) - 1u;
+    for (;;) {
+        const bool want = !hv.held && !hv.done && S.tcnt[rb] < (uint32_t)T;
+        const uint64_t wl = __ballot(want && j == 0u);
+        uint32_t base = 0;
+        if (lane == 0 && wl) base = atomicAdd(cursor, (uint32_t)__popcll(wl));
+        base = bcast0(base);
+        if (want) {
+            const uint32_t idx = base + (uint32_t)__popcll(wl & ((1ull << (G * grp)) - 1ull));
+            if (idx >= size) {
+                hv.done = true;
+            } else {
+                const uint4 e = qc[idx];
+                const M mk = maskc[e.x];
+                hv.pos = e.y;
+                hv.end = e.z;
+                hv.pend = ALL;
+                hv.vm = mk;
+                hv.held = e.y < e.z;
+                if (j == 0u) {
+                    maskc[e.x] = (M)0;  // consumed; this buffer is refilled two levels later
+                    t.checks += (unsigned long long)(e.z - e.y) * O::popc(mk);
+                    t.expansions++;
+                }
+            }
+        }
+        if (!__ballot(hv.held)) {
+            // no group holds edges: pop again while the queue and the table allow
+            if (__ballot(!hv.done && S.tcnt[rb] < (uint32_t)T)) continue;
+            return;
+        }
+        bool failed = false;
+        if (hv.held) {
+            const uint32_t e = hv.pos + j;
+            if (((hv.pend >> j) & 1u) && e < hv.end) {
+                const uint32_t m = (uint32_t)c.ci[e] - c.wb;
+                const uint32_t hs = rtab_find<M>(S, &S.tcnt[rb], m);
+                if (hs == FULLI) failed = true;
+                else atomicOr(&S.hmask[hs], hv.vm);
+            }
+        }
+        const uint32_t fm = grp_ballot<G>(failed, grp);
+        if (hv.held) {
+            if (fm) {
+                hv.pend = fm;  // keep the unplaced edges of this group for the next round
+            } else {
+                hv.pos += G;
+                hv.pend = ALL;
+                hv.held = hv.pos < hv.end;
+            }
+        }
+        if (__ballot(fm != 0u)) return;  // the table is full for everyone
+    }
+}
+
+template <int G, class M>
 __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
     using O = MaskOps<M>;
     using VW = typename O::V;
@@ -146,8 +247,6 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / WAVE;
-    const uint32_t grp = lane / G;
-    const uint32_t j = lane % G;
     const uint32_t W = p.W;
     VW* vis = reinterpret_cast<VW*>(p.vis) + (uint64_t)blockIdx.x * W;
     M* nmask0 = reinterpret_cast<M*>(p.nmask) + (uint64_t)blockIdx.x * 2 * W;
@@ -159,10 +258,9 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
         S.hkey[i] = 0u;
         S.hmask[i] = (M)0;
     }
-    if (tid < 6) S.acc[tid] = 0;
+    if (tid < 8) S.acc[tid] = 0;
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
-        S.cursor = 0;
         S.tcnt[0] = S.tcnt[1] = S.held[0] = S.held[1] = 0;
     }
     uint32_t rb = 0;  // round parity
@@ -170,13 +268,22 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
         if (tid == 0) {
             S.chunk = atomicAdd(&p.ctrl->src_next, 1u);
             S.cte = S.creach = 0;
+            S.cursor[0] = S.cursor[1] = 0;
         }
         if (tid < (uint32_t)O::BITS) S.s_fill[tid] = 0;
         __syncthreads();
         const uint32_t ck = S.chunk;
         if (ck >= p.n_chunks) break;
         const BlkChunk C = p.chunks[ck];
-        const uint32_t wb = p.mat_base[C.mat];
+        RCur c;
+        c.tpass = (C.stride & PASS_T) != 0u;
+        c.rp = c.tpass ? p.rpT : p.rp;
+        c.ci = c.tpass ? p.ciT : p.ci;
+        c.first = C.first;
+        c.count = C.count;
+        c.stride = C.stride & ~PASS_T;
+        c.mat = C.mat;
+        c.wb = p.mat_base[C.mat];
         if (epoch >= p.loops) {  // stamps exhausted: clear this workgroup's visited words
             for (uint32_t i = tid; i < W; i += RB) vis[i] = O::make((M)0, 0u);
             epoch = 0;
@@ -185,92 +292,42 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
         epoch++;
         const uint32_t stamp = epoch;
         // level 0: the sources themselves, each visited by and queued for its own slot
-        if (tid < C.count) {
-            const uint32_t sg = C.first - tid * C.stride;
-            const uint32_t s = sg - wb;
+        if (tid < c.count) {
+            const uint32_t sg = c.first - tid * c.stride;
+            const uint32_t s = sg - c.wb;
             const M bit = (M)1 << tid;
             vis[s] = O::make(bit, stamp);
             nmask0[s] = bit;
-            const uint32_t r0 = (uint32_t)p.rp[sg], r1 = (uint32_t)p.rp[sg + 1];
+            const uint32_t r0 = (uint32_t)c.rp[sg], r1 = (uint32_t)c.rp[sg + 1];
             q0[tid] = make_uint4(s, r0, r1, 0u);
             t.te += r1 - r0;
         }
-        if (tid == 0) S.qcnt[0] = C.count;
+        if (tid == 0) S.qcnt[0] = c.count;
         __syncthreads();
 
         uint32_t cur = 0;
         for (uint32_t lev = 0;; lev++) {
+            // final since the last barrier; reset during this level's relax
             const uint32_t size = S.qcnt[cur];
-            __syncthreads();  // every thread has the size before it can change
             if (size == 0) break;
             const uint32_t nxt = cur ^ 1u;
             M* maskc = nmask0 + (uint64_t)cur * W;
             M* maskn = nmask0 + (uint64_t)nxt * W;
             const uint4* qc = q0 + (uint64_t)cur * W;
             uint4* qn = q0 + (uint64_t)nxt * W;
-            // lane group state (group-uniform): the held vertex's mask, edge
-            // cursor and row end; pend = lanes of the current edge group not yet folded
-            bool held = false, done = false;
-            uint32_t pos = 0, end = 0, pend = 0;
-            M vm = (M)0;
-            for (;;) {  // rounds
-                // ---- A. combine
-                for (;;) {
-                    const bool want = !held && !done && S.tcnt[rb] < (uint32_t)T;
-                    const uint64_t wl = __ballot(want && j == 0u);
-                    uint32_t base = 0;
-                    if (lane == 0 && wl) base = atomicAdd(&S.cursor, (uint32_t)__popcll(wl));
-                    base = bcast0(base);
-                    if (want) {
-                        const uint32_t idx = base + (uint32_t)__popcll(wl & ((1ull << (G * grp)) - 1ull));
-                        if (idx >= size) {
-                            done = true;
-                        } else {
-                            const uint4 e = qc[idx];
-                            const M mk = maskc[e.x];
-                            pos = e.y;
-                            end = e.z;
-                            pend = (1u << G) - 1u;
-                            vm = mk;
-                            held = e.y < e.z;
-                            if (j == 0u) {
-                                maskc[e.x] = (M)0;  // consumed; refilled two levels later
-                                t.checks += (unsigned long long)(e.z - e.y) * O::popc(mk);
-                                t.expansions++;
-                            }
-                        }
-                    }
-                    if (!__ballot(held)) {
-                        // no group holds edges: pop again while the queue and the table allow
-                        if (__ballot(!done && S.tcnt[rb] < (uint32_t)T)) continue;
-                        break;
-                    }
-                    bool failed = false;
-                    if (held) {
-                        const uint32_t e = pos + j;
-                        if (((pend >> j) & 1u) && e < end) {
-                            const uint32_t m = (uint32_t)p.ci[e] - wb;
-                            const uint32_t hs = rtab_find<M>(S, &S.tcnt[rb], m);
-                            if (hs == FULLI) failed = true;
-                            else atomicOr(&S.hmask[hs], vm);
-                        }
-                    }
-                    const uint32_t fm = grp_bits(__ballot(failed), grp);
-                    if (held) {
-                        if (fm) {
-                            pend = fm;  // keep the unplaced edges of this group for the next round
-                        } else {
-                            pos += G;
-                            pend = (1u << G) - 1u;
-                            held = pos < end;
-                        }
-                    }
-                    if (__ballot(fm != 0u)) break;  // the table is full for everyone
-                }
-                if (held && j == 0u) atomicOr(&S.held[rb], 1u);
+            uint32_t* cursor = &S.cursor[lev & 1u];
+            RGrp<M> hv;
+            hv.held = false;
+            hv.done = false;
+            hv.pend = 0u;
+            hv.pos = hv.end = 0u;
+            hv.vm = (M)0;
+            for (;;) {  // rounds, two barriers each
+                rcombine<G, M>(S, hv, c, maskc, qc, size, cursor, rb, t);
+                if (hv.held && (lane % G) == 0u) atomicOr(&S.held[rb], 1u);
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
-                const bool more = S.held[rb] != 0u || S.cursor < size;
+                const bool more = S.held[rb] != 0u || *cursor < size;
                 // ---- B. relax: one lane per table entry
                 const uint32_t tc = S.tcnt[rb];
                 const uint32_t ntab = tc < (uint32_t)H ? tc : (uint32_t)H;
@@ -286,20 +343,20 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                         S.hkey[hs] = 0u;
                         S.hmask[hs] = (M)0;
                     }
-                    const uint32_t mg = m + wb;
-                    M nw = (M)0, bl = (M)0, up = (M)0;
+                    const uint32_t mg = m + c.wb;
+                    M up = (M)0;
                     uint32_t r0 = 0, r1 = 0;
                     bool enq = false;
                     if (ok) {
                         const VW vw = vis[m];
                         const M mn = maskn[m];
-                        r0 = (uint32_t)p.rp[mg];
-                        r1 = (uint32_t)p.rp[mg + 1];
+                        r0 = (uint32_t)c.rp[mg];
+                        r1 = (uint32_t)c.rp[mg + 1];
                         const M vv = O::get(vw, stamp);
-                        nw = mk & ~vv;
+                        const M nw = mk & ~vv;
                         if (nw) {
                             vis[m] = O::make(vv | nw, stamp);
-                            bl = nw & above_mask<M>(mg, C.first, C.stride, C.count);
+                            const M bl = nw & above_mask<M>(mg, c.first, c.stride, c.count);
                             up = lev ? (nw & ~bl) : (M)0;  // level 0: A's own entries
                             if (bl) {
                                 maskn[m] = mn | bl;
@@ -328,13 +385,13 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                             if (lane == 0) fp = aadd64(&p.ctrl->fill_pos, total);
                             fp = bcast0_64(fp) + (incl - nf);
                             for (M b = up; b; b &= b - (M)1, fp++) {
-                                const uint32_t src = C.first - O::ctz(b) * C.stride;
-                                const uint64_t pr = p.transposed ? (((uint64_t)mg << 32) | src)
-                                                                 : (((uint64_t)src << 32) | mg);
+                                const uint32_t src = c.first - O::ctz(b) * c.stride;
+                                const uint64_t pr = c.tpass ? (((uint64_t)mg << 32) | src)
+                                                            : (((uint64_t)src << 32) | mg);
                                 if (fp < p.fill_cap) p.fills[fp] = pr;
                             }
                         } else if (nf) {  // counts only: fills per row
-                            if (p.transposed) {
+                            if (c.tpass) {
                                 aadd64(&p.row_fills[mg], nf);
                             } else {
                                 for (M b = up; b; b &= b - (M)1) atomicAdd(&S.s_fill[O::ctz(b)], 1ull);
@@ -342,25 +399,24 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                         }
                     }
                 }
-                // the next round's counters, cleared while others still relax
-                // (the other parity was last read before the previous barrier)
+                // counters of the next round, the consumed queue and the next
+                // level's cursor, cleared while others still relax: each was
+                // last read before this round's first barrier
                 if (tid == 0) {
                     S.tcnt[rb ^ 1u] = 0;
                     S.held[rb ^ 1u] = 0;
+                    S.qcnt[cur] = 0;
+                    S.cursor[(lev + 1u) & 1u] = 0;
                 }
                 __syncthreads();
                 rb ^= 1u;
                 if (!more) break;
             }
-            if (tid == 0) {
-                S.qcnt[cur] = 0;
-                S.cursor = 0;
-            }
-            __syncthreads();
             cur = nxt;
             levels++;
         }
-        // the mini-group's TE and |R| (per matrix); counts-only fills per source row
+        // the mini-group's TE and |R| (per matrix for the A pass); counts-only
+        // fills per source row
         const unsigned long long te = wave_sum64(t.te), rc = wave_sum64(t.reach);
         if (lane == 0 && (te | rc)) {
             atomicAdd(&S.cte, te);
@@ -369,34 +425,31 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
         t.te = t.reach = 0;
         __syncthreads();
         if (tid == 0) {
-            atomicAdd(&S.acc[1], S.cte);
-            atomicAdd(&S.acc[2], S.creach);
-            if (!p.transposed && p.mat_te) {
-                aadd64(&p.mat_te[C.mat], S.cte);
-                aadd64(&p.mat_reach[C.mat], S.creach);
+            atomicAdd(&S.acc[c.tpass ? 6 : 4], S.cte);
+            atomicAdd(&S.acc[c.tpass ? 7 : 5], S.creach);
+            if (!c.tpass && p.mat_te) {
+                aadd64(&p.mat_te[c.mat], S.cte);
+                aadd64(&p.mat_reach[c.mat], S.creach);
             }
         }
-        if (p.fill_cap == 0ull && !p.transposed && tid < C.count)
-            p.row_fills[C.first - tid * C.stride] = S.s_fill[tid];
+        if (p.fill_cap == 0ull && !c.tpass && tid < c.count && S.s_fill[tid])
+            aadd64(&p.row_fills[c.first - tid * c.stride], S.s_fill[tid]);
         __syncthreads();
     }
     if (t.fills) atomicAdd(&S.acc[0], t.fills);
-    if (t.checks) atomicAdd(&S.acc[3], t.checks);
-    if (t.expansions) atomicAdd(&S.acc[5], t.expansions);
+    if (t.checks) atomicAdd(&S.acc[1], t.checks);
+    if (t.expansions) atomicAdd(&S.acc[3], t.expansions);
     __syncthreads();
     if (tid == 0) {
-        Ctrl* c = p.ctrl;
-        if (S.acc[0]) aadd64(&c->fill_count, S.acc[0]);
-        if (p.transposed) {
-            if (S.acc[1]) aadd64(&c->te_t, S.acc[1]);
-            if (S.acc[2]) aadd64(&c->reach_t, S.acc[2]);
-        } else {
-            if (S.acc[1]) aadd64(&c->te, S.acc[1]);
-            if (S.acc[2]) aadd64(&c->reach, S.acc[2]);
-        }
-        if (S.acc[3]) aadd64(&c->edge_checks, S.acc[3]);
-        if (S.acc[5]) aadd64(&c->expansions, S.acc[5]);
-        aadd64(&c->levels, levels);
+        Ctrl* ct = p.ctrl;
+        if (S.acc[0]) aadd64(&ct->fill_count, S.acc[0]);
+        if (S.acc[1]) aadd64(&ct->edge_checks, S.acc[1]);
+        if (S.acc[3]) aadd64(&ct->expansions, S.acc[3]);
+        if (S.acc[4]) aadd64(&ct->te, S.acc[4]);
+        if (S.acc[5]) aadd64(&ct->reach, S.acc[5]);
+        if (S.acc[6]) aadd64(&ct->te_t, S.acc[6]);
+        if (S.acc[7]) aadd64(&ct->reach_t, S.acc[7]);
+        aadd64(&ct->levels, levels);
         p.bstate[blockIdx.x] = epoch;
     }
 }
@@ -406,9 +459,10 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
 int rch_block_size() { return RB; }
 bool rch_bits_valid(int bits) { return bits == 32 || bits == 64; }
 const void* rch_kernel_ptr(int bits) {
-    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<uint64_t>)
-                      : reinterpret_cast<const void*>(&k_reach_blk<uint32_t>);
+    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<8, uint64_t>)
+                      : reinterpret_cast<const void*>(&k_reach_blk<8, uint32_t>);
 }
+uint32_t rch_pass_bit() { return PASS_T; }
 // Per workgroup: visited words (8 or 16 B), two next-level masks, two
 // 16-byte frontier queues, per vertex of the window.
 int64_t reach_bytes(uint32_t nb, uint32_t W, int bits) {
@@ -417,8 +471,8 @@ int64_t reach_bytes(uint32_t nb, uint32_t W, int bits) {
 }
 
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s) {
-    if (bits == 64) hipLaunchKernelGGL(k_reach_blk<uint64_t>, dim3(p.nblocks), dim3(RB), 0, s, p);
-    else if (bits == 32) hipLaunchKernelGGL(k_reach_blk<uint32_t>, dim3(p.nblocks), dim3(RB), 0, s, p);
+    if (bits == 64) hipLaunchKernelGGL((k_reach_blk<8, uint64_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
+    else if (bits == 32) hipLaunchKernelGGL((k_reach_blk<8, uint32_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
