@@ -21,10 +21,11 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import GSOFA_FLAG_COUNTS_ONLY, GSOFA_FLAG_SPILL
+from ._lib import GSOFA_FLAG_COUNTS_ONLY, GSOFA_FLAG_GRAPH, GSOFA_FLAG_SPILL
 
 __all__ = ["CsrMatrix", "SymbfactResult", "GsofaError", "Plan", "Server", "symbfact", "gen_grid",
-           "gen_band", "read_ascii", "write_ascii", "footprint", "GSOFA_FLAG_COUNTS_ONLY", "GSOFA_FLAG_SPILL"]
+           "gen_band", "read_ascii", "write_ascii", "footprint", "GSOFA_FLAG_COUNTS_ONLY", "GSOFA_FLAG_SPILL",
+           "GSOFA_FLAG_GRAPH"]
 
 
 class GsofaError(RuntimeError):
@@ -189,7 +190,7 @@ def write_ascii(A, colbeg_path, colend_path, csr_path):
 def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
              max_supernode_size=0, chunk=0, wave_block_threshold=0, device=0, queue_capacity=0,
              output_capacity=0, counts_only=False, grid_blocks=0, mode=0, top_sources=0,
-             block_slots=0, spill=False):
+             block_slots=0, spill=False, graph=False):
     o = _lib.Options()
     _lib.lib().gsofa_options_default(C.byref(o))
     o.n_virtual_gpu = n_virtual_gpu
@@ -202,7 +203,8 @@ def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
     o.device = device
     o.queue_capacity = queue_capacity
     o.output_capacity = output_capacity
-    o.flags = (GSOFA_FLAG_COUNTS_ONLY if counts_only else 0) | (GSOFA_FLAG_SPILL if spill else 0)
+    o.flags = ((GSOFA_FLAG_COUNTS_ONLY if counts_only else 0) | (GSOFA_FLAG_SPILL if spill else 0) |
+               (GSOFA_FLAG_GRAPH if graph else 0))
     o.grid_blocks = grid_blocks
     o.mode = {"auto": 0, "block": 1, "grid": 2, "reach": 4}.get(mode, mode)
     o.top_sources = top_sources
@@ -232,6 +234,7 @@ class SymbfactResult:
         self.rescans = int(r.rescans)
         self.n_super = int(r.n_super)
         self.passes = int(r.passes)
+        self.graph_replay = bool(r.graph_replay)
 
         def arr(ptr, cnt, ct, dt):
             if not ptr or cnt <= 0:

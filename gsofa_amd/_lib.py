@@ -12,7 +12,8 @@ LIB_PATH = os.path.join(_HERE, "libgsofa.so")
 
 GSOFA_FLAG_COUNTS_ONLY = 0x1
 GSOFA_FLAG_SPILL = 0x2
-MODE_AUTO, MODE_BLOCK, MODE_GRID, MODE_HYBRID = 0, 1, 2, 3
+GSOFA_FLAG_GRAPH = 0x4
+MODE_AUTO, MODE_BLOCK, MODE_GRID, MODE_HYBRID, MODE_REACH = 0, 1, 2, 3, 4
 
 # Every entry point include/gsofa.h declares (kept in sync by a CPU test).
 EXPORTS = [
@@ -60,7 +61,8 @@ class Result(C.Structure):
                 ("share_ms_max", C.c_double), ("chunk", C.c_int32), ("mode", C.c_int32),
                 ("grid_blocks", C.c_int32),
                 ("queue_capacity", C.c_int64), ("levels", C.c_uint64), ("rescans", C.c_uint64),
-                ("row_te", C.c_void_p), ("row_reach", C.c_void_p), ("passes", C.c_int32)]
+                ("row_te", C.c_void_p), ("row_reach", C.c_void_p), ("passes", C.c_int32),
+                ("graph_replay", C.c_int32)]
 
 
 class Supernodal(C.Structure):

@@ -144,6 +144,14 @@ struct gsofa_plan {
     bool host_valid = false;
     std::vector<hipEvent_t> ev;
     uint64_t spin_ticks = 0;
+    // GSOFA_FLAG_GRAPH: the captured execute (traversal, structure,
+    // supernodes, counters to pinned host memory), replayed by later executes
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_failed = false;  // capture was refused: executes run eagerly
+    bool capturing = false;     // the stream is being captured: no event records
+    Ctrl* h_ctrl = nullptr;     // pinned copy of the device counters
+    bool last_replayed = false; // the last execute replayed the graph
 };
 
 namespace {
@@ -1221,9 +1229,9 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         if (!P->structure)  // both passes add their fills per row
             HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_row_fk), 2 * (uint64_t)n, 0u, P->stream));
         HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
-        HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+        if (!P->capturing) HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
         if (rp.n_chunks > 0) HIPCHK(launch_reach(rp, P->rbits, P->stream));
-        HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+        if (!P->capturing) HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
         return GSOFA_OK;
     }
     if (P->mode == GSOFA_MODE_BLOCK) {
@@ -1248,7 +1256,7 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         bp.ctrl = P->d_ctrl;
         bp.bstate = P->d_bstate;
         HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
-        HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
+        if (!P->capturing) HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
         if (P->hybrid && P->h_g_hi > P->h_g_lo) {  // the top groups, whole device each
             TravParams tp = grid_params(P, gid);
             tp.chunk = P->hC;
@@ -1262,7 +1270,7 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
             advance_groups(P, gid, P->hC, (uint64_t)P->h_g_lo, (uint64_t)P->h_g_hi);
         }
         if (bp.n_chunks > 0) HIPCHK(launch_traverse_blk(bp, P->S, P->blk_lds, P->stream));
-        HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
+        if (!P->capturing) HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
         return GSOFA_OK;
     }
     TravParams tp = grid_params(P, gid);
@@ -1492,6 +1500,60 @@ static void finish_timing(gsofa_plan* P, size_t nshares) {
     P->ms = ev_ms(P->ev[0], P->ev[1]);
 }
 
+// The device work of one bounded execute, queued on the plan's stream:
+// counters cleared, every share's traversal, the structure and supernode
+// passes, and the counters copied to `stats` (host memory).  Nothing here
+// waits for the device, allocates or sets memory with a memset node, so the
+// sequence can be captured in a graph.
+static int queue_bounded(gsofa_plan* P, const std::vector<int32_t>& shares, int32_t N, bool all_rows,
+                         Ctrl* stats) {
+    HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_ctrl), sizeof(Ctrl) / 4, 0u, P->stream));
+    if (!P->capturing) HIPCHK(hipEventRecord(P->ev[0], P->stream));
+    for (size_t si = 0; si < shares.size(); si++) {
+        int rc = run_share(P, shares[si], si);
+        if (rc) return rc;
+    }
+    if (P->structure) {
+        int rc = build_structure(P, 0, N, all_rows ? -1 : P->opt.virtual_gpu_id, true);
+        if (!rc && P->supernodes) rc = build_supernodes(P, nullptr);
+        if (rc) return rc;
+    }
+    HIPCHK(hipMemcpyAsync(stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
+    if (!P->capturing) HIPCHK(hipEventRecord(P->ev[1], P->stream));
+    return GSOFA_OK;
+}
+
+// GSOFA_FLAG_GRAPH: capture queue_bounded once (after an eager execute has
+// allocated every lazily sized buffer).  A refused capture leaves the plan
+// executing eagerly.
+static void capture_graph(gsofa_plan* P, const std::vector<int32_t>& shares, int32_t N, bool all_rows) {
+    if (!P->h_ctrl && hipHostMalloc((void**)&P->h_ctrl, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        P->h_ctrl = nullptr;
+        P->graph_failed = true;
+        return;
+    }
+    if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        (void)hipGetLastError();
+        P->graph_failed = true;
+        return;
+    }
+    P->capturing = true;
+    const int rc = queue_bounded(P, shares, N, all_rows, P->h_ctrl);
+    P->capturing = false;
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(P->stream, &g);
+    if (rc != GSOFA_OK || e != hipSuccess || !g ||
+        hipGraphInstantiate(&P->graph_exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        if (g) (void)hipGraphDestroy(g);
+        P->graph_exec = nullptr;
+        P->graph_failed = true;
+        return;
+    }
+    P->graph = g;
+}
+
 extern "C" {
 
 int gsofa_plan_execute(gsofa_plan* P) {
@@ -1509,6 +1571,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
     P->spilled = false;
     P->executed = false;
     P->host_valid = false;
+    P->last_replayed = false;
     memset(&P->stats, 0, sizeof(P->stats));
     if (n == 0) {
         P->executed = true;
@@ -1528,28 +1591,40 @@ int gsofa_plan_execute(gsofa_plan* P) {
     }
 
     P->passes = 0;
+    // graph replays: the modes whose launches carry no per-execute host state
+    const bool graphable = (P->opt.flags & GSOFA_FLAG_GRAPH) && P->bounded && !P->graph_failed &&
+                           (P->mode == GSOFA_MODE_REACH || (P->mode == GSOFA_MODE_BLOCK && !P->hybrid));
+    if (graphable && P->graph_exec) {
+        P->passes = 1;
+        HIPCHK(hipEventRecord(P->ev[0], P->stream));
+        HIPCHK(hipGraphLaunch(P->graph_exec, P->stream));
+        HIPCHK(hipEventRecord(P->ev[1], P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        P->stats = *P->h_ctrl;
+        if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
+        if (!P->structure || P->stats.fill_pos <= P->fill_cap) {
+            // the graph holds no events of its own: its whole replay is timed
+            P->ms = ev_ms(P->ev[0], P->ev[1]);
+            P->traverse_ms = P->share_min = P->share_max = 0;
+            P->totals_known = !P->structure;
+            P->last_replayed = true;
+            P->executed = true;
+            return GSOFA_OK;
+        }
+        return GSOFA_ERR_STATE;  // cannot happen: the bounds hold
+    }
     if (P->bounded) {
         P->passes = 1;
         // one pass: traversal, structure and supernodes are queued back to
         // back; the host waits once, at the end
-        HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_ctrl), sizeof(Ctrl) / 4, 0u, P->stream));
-        HIPCHK(hipEventRecord(P->ev[0], P->stream));
-        for (size_t si = 0; si < shares.size(); si++) {
-            int rc = run_share(P, shares[si], si);
-            if (rc) return rc;
-        }
-        if (P->structure) {
-            int rc = build_structure(P, 0, N, all_rows ? -1 : P->opt.virtual_gpu_id, true);
-            if (!rc && P->supernodes) rc = build_supernodes(P, nullptr);
-            if (rc) return rc;
-        }
-        HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipEventRecord(P->ev[1], P->stream));
+        int rc = queue_bounded(P, shares, N, all_rows, &P->stats);
+        if (rc) return rc;
         HIPCHK(hipStreamSynchronize(P->stream));
         if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
         if (!P->structure || P->stats.fill_pos <= P->fill_cap) {
             finish_timing(P, shares.size());
             P->executed = true;
+            if (graphable) capture_graph(P, shares, N, all_rows);
             return GSOFA_OK;
         }
         // the bounds do not hold (cannot happen for a valid matrix): take the
@@ -1604,6 +1679,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->levels = P->stats.levels;
     r->rescans = P->stats.rescans;
     r->passes = P->passes;
+    r->graph_replay = P->last_replayed ? 1 : 0;
     r->ms = P->ms;
     r->traverse_ms = P->traverse_ms;
     r->share_ms_min = P->share_min;
@@ -1764,6 +1840,7 @@ int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
         r->rescans = G.rescans;
     }
     r->passes = G.passes;
+    r->graph_replay = G.graph_replay;
     if (!G.row_fill) return GSOFA_OK;  // counts only in grid or hybrid mode: totals alone
     const bool lu = G.L_rowptr != nullptr;  // false: counts only (per-row fills, TE, |R|)
     r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
@@ -1831,6 +1908,9 @@ void gsofa_plan_destroy(gsofa_plan* P) {
     if (!P) return;
     (void)hipSetDevice(P->device);
     (void)hipStreamSynchronize(P->stream);
+    if (P->graph_exec) (void)hipGraphExecDestroy(P->graph_exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->h_ctrl) (void)hipHostFree(P->h_ctrl);
     for (auto e : P->ev) (void)hipEventDestroy(e);
     gsofa_result_free(&P->host);
     release_device(P);

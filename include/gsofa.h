@@ -73,6 +73,16 @@ void gsofa_csr_free(gsofa_csr* m);
                                     /* n_virtual_gpu shares, move its L/U rows  */
                                     /* to host memory, so device output holds   */
                                     /* one share (supernodes stay on device)    */
+#define GSOFA_FLAG_GRAPH 0x4        /* plans: the first execute runs and then   */
+                                    /* captures its device work (traversal,     */
+                                    /* structure, supernodes) in a HIP graph;   */
+                                    /* later executes replay it, one launch and */
+                                    /* one host wait each.  Block and reach     */
+                                    /* modes with output sized from the row     */
+                                    /* bounds; otherwise it is ignored.  No     */
+                                    /* counterpart in the reference, whose host */
+                                    /* relaunches every level (with_space_      */
+                                    /* optimization/symbfact.cuh:1247-1311)     */
 
 typedef struct gsofa_options {
     int32_t n_virtual_gpu;        /* sources split cyclically into this many shares (>=1) */
@@ -154,6 +164,9 @@ typedef struct gsofa_result {
     int64_t* row_reach;   /* n: |R_<(s)| of each processed row (block mode only)       */
     int32_t passes;       /* traversal passes of the execute: 1, or more when an       */
                           /* explicit output_capacity overflowed and the run repeated  */
+    int32_t graph_replay; /* 1: the execute replayed the plan's captured graph          */
+                          /* (GSOFA_FLAG_GRAPH); ms then times the whole replay and     */
+                          /* traverse_ms is 0                                           */
 } gsofa_result;
 
 void gsofa_result_free(gsofa_result* r);
