@@ -104,6 +104,16 @@ def batch_bench(steps, warmup):
             times.append(time.perf_counter() - t0)
         fills = [p.fetch_matrix(b).fill_count for b in range(len(mats))]
     batched_s = sum(times) / len(times)
+    with g.Plan(mats, graph=True) as p:
+        p.execute()  # runs, then captures the graph
+        gtimes = []
+        for _ in range(max(steps, 1)):
+            t0 = time.perf_counter()
+            p.execute()
+            gtimes.append(time.perf_counter() - t0)
+        gfills = [p.fetch_matrix(b).fill_count for b in range(len(mats))]
+        replayed = p.fetch_matrix(0).graph_replay
+    graph_s = sum(gtimes) / len(gtimes)
     lat = []
     for A in mats:
         with g.Plan(A) as q:
@@ -116,12 +126,14 @@ def batch_bench(steps, warmup):
     if os.path.exists(gpath):
         gold = json.load(open(gpath)).get("grid2d_nd_batch")
         if gold:
-            gold_ok = [x["fill_count"] for x in gold] == fills
+            gold_ok = [x["fill_count"] for x in gold] == fills == gfills
     lat_ms = np.array(lat) * 1e3
     return {"workload": "256 ND 2D grids, side log-uniform [16,256], seed 4",
             "matrices": len(mats), "vertices": int(sum(A.n for A in mats)),
             "batched_ms": round(batched_s * 1e3, 3),
             "batched_matrices_per_s": round(len(mats) / batched_s, 1),
+            "batched_graph_ms": round(graph_s * 1e3, 3), "graph_replayed": bool(replayed),
+            "batched_graph_matrices_per_s": round(len(mats) / graph_s, 1),
             "one_by_one_ms": round(obo_s * 1e3, 3),
             "one_by_one_matrices_per_s": round(len(mats) / obo_s, 1),
             "speedup_vs_one_by_one": round(obo_s / batched_s, 2),
@@ -130,55 +142,74 @@ def batch_bench(steps, warmup):
             "matches_golden": gold_ok}
 
 
-def serve_bench(count, rate, inflight=2, max_batch=64):
-    """Serving: a seeded Poisson stream (rate req/s) of the batch family's 2D grids and 3D
-    7-point ND grids (side 16..64), submitted at its arrival times to the
-    request server (continuous batching, `inflight` batches on the device) and,
-    as the one-by-one baseline, to a server that runs one request per launch.
-    Latency = submit -> result on the host (includes the upload and the copy
-    of the result)."""
+def serve_bench(count, rates, inflight=4, max_batch=64):
+    """Serving: seeded Poisson streams (one per rate, req/s) of the batch
+    family's 2D grids and 3D 7-point ND grids (side 16..64), submitted at their
+    arrival times to the request server (continuous batching: `inflight`
+    workers, each with a device-resident arena) and, as the one-by-one
+    baseline, run through the plain one-matrix path (gsofa_symbfact: plan,
+    execute, fetch, release) in arrival order.  Latency = arrival -> result on
+    the host, as numpy arrays (both paths include the same conversion)."""
     import numpy as np
 
     import gsofa_amd as g
     import gsofa_amd.configs as cf
-    reqs = cf.stream(count, rate)
 
-    def run(srv):
-        srv.wait(srv.submit(cf.batch_matrix(16, 0)))  # warm-up
+    def served(srv, reqs):
+        # clients submit at their arrival times from a few threads (a submit
+        # packs its matrix in the caller's thread), as independent callers would
+        from concurrent.futures import ThreadPoolExecutor
         t0 = time.perf_counter()
-        tickets = []
+        futs = []
+        with ThreadPoolExecutor(max_workers=8) as pool:
+            for ta, _, _, A in reqs:
+                now = time.perf_counter() - t0
+                if ta > now:
+                    time.sleep(ta - now)
+                futs.append(pool.submit(srv.submit, A))
+            tickets = [f.result() for f in futs]
+        out = [srv.wait(t) for t in tickets]
+        span = time.perf_counter() - t0
+        lat = [i["latency_ms"] + 0.0 for _, i in out]
+        return [r for r, _ in out], lat, span, len({i["batch_id"] for _, i in out})
+
+    def one_by_one(reqs):
+        t0 = time.perf_counter()
+        res, lat = [], []
         for ta, _, _, A in reqs:
             now = time.perf_counter() - t0
             if ta > now:
                 time.sleep(ta - now)
-            tickets.append(srv.submit(A))
-        out = [srv.wait(t) for t in tickets]
-        return out, time.perf_counter() - t0
+            res.append(g.symbfact(A))
+            lat.append((time.perf_counter() - t0 - ta) * 1e3)
+        return res, lat, time.perf_counter() - t0
 
-    with g.Server(max_inflight=inflight, max_batch=max_batch, counts_only=False) as srv:
-        served, span = run(srv)
-    with g.Server(max_inflight=1, max_batch=1) as srv:
-        alone, span1 = run(srv)
-    same = all(a.fill_count == b.fill_count and a.te_alg == b.te_alg and
-               np.array_equal(a.U_col, b.U_col) and np.array_equal(a.L_col, b.L_col)
-               for (a, _), (b, _) in zip(served, alone))
-    lat = np.array([i["latency_ms"] for _, i in served])
-    lat1 = np.array([i["latency_ms"] for _, i in alone])
-    te = sum(r.te_alg for r, _ in served)
-    return {"workload": "Poisson stream, %d requests at %.0f req/s, seed 5: 2D ND grids (side 16..256) and "
-                        "3D 7-point ND grids (side 16..64)" % (count, rate),
-            "requests": count, "kinds_3d": sum(k for _, k, _, _ in reqs),
-            "max_inflight": inflight, "max_batch": max_batch,
-            "batches": len({i["batch_id"] for _, i in served}),
-            "served_s": round(span, 3), "served_matrices_per_s": round(count / span, 2),
-            "served_gteps": round(te / span / 1e9, 2),
-            "latency_ms_p50": round(float(np.percentile(lat, 50)), 2),
-            "latency_ms_p99": round(float(np.percentile(lat, 99)), 2),
-            "one_by_one_s": round(span1, 3), "one_by_one_matrices_per_s": round(count / span1, 2),
-            "one_by_one_latency_ms_p50": round(float(np.percentile(lat1, 50)), 2),
-            "one_by_one_latency_ms_p99": round(float(np.percentile(lat1, 99)), 2),
-            "speedup_vs_one_by_one": round(span1 / span, 2),
-            "results_equal_alone": bool(same)}
+    out = {"workload": "Poisson streams, %d requests each, seed 5: 2D ND grids (side 16..256) and 3D 7-point "
+                       "ND grids (side 16..64)" % count,
+           "requests": count, "max_inflight": inflight, "max_batch": max_batch, "rates": []}
+    g.symbfact(cf.batch_matrix(16, 0))  # warm-up of the one-shot path
+    with g.Server(max_inflight=inflight, max_batch=max_batch) as srv:
+        srv.wait(srv.submit(cf.batch_matrix(16, 0)))  # warm-up
+        for rate in rates:
+            reqs = cf.stream(count, rate)
+            sres, slat, sspan, nb = served(srv, reqs)
+            ores, olat, ospan = one_by_one(reqs)
+            same = all(a.fill_count == b.fill_count and a.te_alg == b.te_alg and
+                       np.array_equal(a.U_col, b.U_col) and np.array_equal(a.L_col, b.L_col) and
+                       np.array_equal(a.super_start, b.super_start) for a, b in zip(sres, ores))
+            te = sum(r.te_alg for r in sres)
+            rec = {"rate": rate, "kinds_3d": sum(k for _, k, _, _ in reqs), "batches": nb,
+                   "served_s": round(sspan, 3), "served_matrices_per_s": round(count / sspan, 2),
+                   "served_gteps": round(te / sspan / 1e9, 2),
+                   "latency_ms_p50": round(float(np.percentile(slat, 50)), 2),
+                   "latency_ms_p99": round(float(np.percentile(slat, 99)), 2),
+                   "one_by_one_s": round(ospan, 3), "one_by_one_matrices_per_s": round(count / ospan, 2),
+                   "one_by_one_latency_ms_p50": round(float(np.percentile(olat, 50)), 2),
+                   "one_by_one_latency_ms_p99": round(float(np.percentile(olat, 99)), 2),
+                   "speedup_vs_one_by_one": round(ospan / sspan, 2), "results_equal_one_by_one": bool(same)}
+            log("serve rate %g:" % rate, json.dumps(rec))
+            out["rates"].append(rec)
+    return out
 
 
 def main():
@@ -195,7 +226,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-matrix batch leg")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-stream leg")
     ap.add_argument("--serve-count", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=500.0)
+    ap.add_argument("--serve-rates", default="50,200,500,1000,2000",
+                    help="arrival rates (req/s) of the request-stream leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -270,7 +302,7 @@ def main():
         out["batch"] = batch_bench(args.steps, args.warmup)
         log("batch:", json.dumps(out["batch"]))
     if not args.no_serve:
-        out["serve"] = serve_bench(args.serve_count, args.serve_rate)
+        out["serve"] = serve_bench(args.serve_count, [float(x) for x in args.serve_rates.split(",")])
         log("serve:", json.dumps(out["serve"]))
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)

@@ -212,10 +212,39 @@ def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
     return o
 
 
-class SymbfactResult:
-    """Sorted strictly-lower (L) / strictly-upper (U) structure of L+U by row."""
+class _Owner:
+    """Holds a library-owned gsofa_result and releases it (gsofa_result_free)
+    when the last array viewing its memory is gone."""
 
     def __init__(self, r):
+        self.r = r
+
+    def __del__(self):
+        if self.r is not None:
+            try:
+                _lib.lib().gsofa_result_free(C.byref(self.r))
+            except Exception:  # interpreter shutdown: the process releases it
+                pass
+            self.r = None
+
+
+class _View:
+    """numpy view of `cnt` elements at `ptr`, keeping `owner` alive."""
+
+    def __init__(self, ptr, cnt, dt, owner):
+        self.owner = owner
+        self.__array_interface__ = {"data": (int(ptr), False), "shape": (int(cnt),),
+                                    "typestr": np.dtype(dt).str, "version": 3}
+
+
+class SymbfactResult:
+    """Sorted strictly-lower (L) / strictly-upper (U) structure of L+U by row.
+
+    Takes ownership of the library result `r`: the arrays are views of its
+    memory (no copy), released with the last of them."""
+
+    def __init__(self, r):
+        owner = _Owner(r)
         n = int(r.n)
         self.n = n
         self.fill_count = int(r.fill_count)
@@ -239,7 +268,7 @@ class SymbfactResult:
         def arr(ptr, cnt, ct, dt):
             if not ptr or cnt <= 0:
                 return np.zeros(max(cnt, 0), dt) if ptr or cnt == 0 else None
-            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(ct)), shape=(cnt,)).copy()
+            return np.asarray(_View(ptr, cnt, dt, owner))
         self.row_fill = arr(r.row_fill, n, C.c_int64, np.int64) if r.row_fill else None
         self.row_te = arr(r.row_te, n, C.c_int64, np.int64) if r.row_te else None
         self.row_reach = arr(r.row_reach, n, C.c_int64, np.int64) if r.row_reach else None
@@ -286,10 +315,7 @@ def symbfact(A, **kw):
     o = _options(**kw)
     r = _lib.Result()
     _check(_lib.lib().gsofa_symbfact(C.byref(m), C.byref(o), C.byref(r)), "symbfact")
-    try:
-        return SymbfactResult(r)
-    finally:
-        _lib.lib().gsofa_result_free(C.byref(r))
+    return SymbfactResult(r)
 
 
 def footprint(A, **kw):
@@ -321,10 +347,7 @@ class Plan:
     def fetch_matrix(self, index):
         r = _lib.Result()
         _check(_lib.lib().gsofa_plan_fetch_matrix(self._h, index, C.byref(r)), "plan_fetch_matrix")
-        try:
-            return SymbfactResult(r)
-        finally:
-            _lib.lib().gsofa_result_free(C.byref(r))
+        return SymbfactResult(r)
 
     def execute(self):
         _check(_lib.lib().gsofa_plan_execute(self._h), "plan_execute")
@@ -332,10 +355,7 @@ class Plan:
     def fetch(self):
         r = _lib.Result()
         _check(_lib.lib().gsofa_plan_fetch(self._h, C.byref(r)), "plan_fetch")
-        try:
-            return SymbfactResult(r)
-        finally:
-            _lib.lib().gsofa_result_free(C.byref(r))
+        return SymbfactResult(r)
 
     def fetch_supernodal(self):
         """SuperLU_DIST-style hand-off of the last execute (single matrix,
@@ -392,10 +412,7 @@ class Server:
 
     def _final(self, rc, r, info):
         _check(rc, "server request")
-        try:
-            res = SymbfactResult(r)
-        finally:
-            _lib.lib().gsofa_result_free(C.byref(r))
+        res = SymbfactResult(r)
         return res, {"queued_ms": info.queued_ms, "latency_ms": info.latency_ms,
                      "batch_ms": info.batch_ms, "batch_id": info.batch_id,
                      "batch_size": info.batch_size, "spill_shares": info.spill_shares}

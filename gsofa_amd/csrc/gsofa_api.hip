@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "gsofa.h"
@@ -152,6 +153,15 @@ struct gsofa_plan {
     bool capturing = false;     // the stream is being captured: no event records
     Ctrl* h_ctrl = nullptr;     // pinned copy of the device counters
     bool last_replayed = false; // the last execute replayed the graph
+    // arena (request server worker): capacities of the buffers a batch
+    // fills, host copies the uploads read from, and supernode arrays' size
+    bool arena = false;
+    struct Caps {
+        size_t rp = 0, ci = 0, fills = 0, L = 0, U = 0, rows = 0, scan = 0, mats = 0, chunks = 0, wg = 0, nb = 0;
+    } cap;
+    size_t sn_cap = 0;
+    std::vector<int32_t> h_ci, h_rpT, h_ciT;
+    int64_t held = 0;  // device bytes the arena holds
 };
 
 namespace {
@@ -515,6 +525,79 @@ int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
         P->Ucap = cap;
     }
     return GSOFA_OK;
+}
+
+// Device-to-host copies of large results go through pinned staging buffers
+// (a process-wide pool, kept until exit): the DMA of one chunk overlaps the
+// host copy of the previous one into the caller's (pageable) array, where a
+// plain copy to pageable memory moves a few GB/s.
+constexpr size_t kStageBytes = 32u << 20;
+constexpr size_t kStageMin = 8u << 20;  // smaller copies go direct
+std::mutex g_stage_mu;
+std::vector<void*> g_stage_free;
+
+void* stage_get() {
+    {
+        std::lock_guard<std::mutex> lk(g_stage_mu);
+        if (!g_stage_free.empty()) {
+            void* p = g_stage_free.back();
+            g_stage_free.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    if (hipHostMalloc(&p, kStageBytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
+    }
+    return p;
+}
+
+void stage_put(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    g_stage_free.push_back(p);
+}
+
+// dst <- src (device), complete on return.  Work queued on s before the call
+// is ordered before the copy.
+hipError_t copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    void* b[2] = {nullptr, nullptr};
+    if (bytes >= kStageMin) {
+        b[0] = stage_get();
+        b[1] = stage_get();
+    }
+    if (!b[0] || !b[1]) {  // small, or no pinned memory: one direct copy
+        stage_put(b[0]);
+        stage_put(b[1]);
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
+    const size_t nch = (bytes + kStageBytes - 1) / kStageBytes;
+    auto issue = [&](size_t k) {
+        const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+        hipError_t r = hipMemcpyAsync(b[k & 1], (const char*)src + off, len, hipMemcpyDeviceToHost, s);
+        return r == hipSuccess ? hipEventRecord(ev[k & 1], s) : r;
+    };
+    if (e == hipSuccess) e = issue(0);
+    for (size_t k = 0; k < nch && e == hipSuccess; k++) {
+        if (k + 1 < nch) e = issue(k + 1);  // its buffer was emptied in iteration k - 1
+        if (e == hipSuccess) e = hipEventSynchronize(ev[k & 1]);
+        if (e == hipSuccess) {
+            const size_t off = k * kStageBytes, len = std::min(kStageBytes, bytes - off);
+            memcpy((char*)dst + off, b[k & 1], len);
+        }
+    }
+    if (e != hipSuccess) (void)hipStreamSynchronize(s);  // no copy left in flight into the buffers
+    for (hipEvent_t x : ev)
+        if (x) (void)hipEventDestroy(x);
+    stage_put(b[0]);
+    stage_put(b[1]);
+    return e;
 }
 
 float ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -1347,7 +1430,12 @@ static int read_totals(gsofa_plan* P) {
 // per-row pass-through flags.
 static int build_supernodes(gsofa_plan* P, const uint32_t* pass) {
     const int32_t n = P->n;
-    if (!P->d_supno) {
+    if (!P->d_supno || P->sn_cap < (size_t)n) {
+        dfree(P->d_supno);
+        dfree(P->d_sstart);
+        dfree(P->d_nsuper);
+        dfree(P->d_sn64);
+        dfree(P->d_snflag);
         int e = 0;
         e |= dalloc(&P->d_supno, (size_t)n);
         e |= dalloc(&P->d_sstart, (size_t)n + 1);
@@ -1355,6 +1443,7 @@ static int build_supernodes(gsofa_plan* P, const uint32_t* pass) {
         e |= dalloc(&P->d_sn64, scan_tmp_words(n) + 2 * (size_t)n + 8);
         e |= dalloc(&P->d_snflag, (size_t)n);
         if (e) return GSOFA_ERR_ALLOC;
+        P->sn_cap = (size_t)n;
     }
     SuperParams sp{};
     sp.n = n;
@@ -1778,12 +1867,8 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     } else {
         HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
         HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
-        if (P->nnzL)
-            HIPCHK(hipMemcpyAsync(r->L_col, P->d_Lcol, sizeof(int32_t) * P->nnzL, hipMemcpyDeviceToHost,
-                                  P->stream));
-        if (P->nnzU)
-            HIPCHK(hipMemcpyAsync(r->U_col, P->d_Ucol, sizeof(int32_t) * P->nnzU, hipMemcpyDeviceToHost,
-                                  P->stream));
+        HIPCHK(copy_d2h(r->L_col, P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL, P->stream));
+        HIPCHK(copy_d2h(r->U_col, P->d_Ucol, sizeof(int32_t) * (size_t)P->nnzU, P->stream));
     }
     if (r->row_te) {
         HIPCHK(hipStreamSynchronize(P->stream));
@@ -2039,3 +2124,368 @@ int gsofa_symbfact(const gsofa_matrix* A, const gsofa_options* opt, gsofa_result
 }
 
 }  // extern "C"
+
+// ---- arenas ----------------------------------------------------------------
+// A reach-mode plan that takes one batch after another: the request server's
+// workers each hold one.  Its buffers grow to the largest batch seen and are
+// not freed between batches, so a worker in steady state makes no device
+// allocation (a hipFree waits for the whole device, which would stall the
+// other workers' batches).  The per-workgroup arrays keep one stride (their
+// capacity), so a workgroup's visited stamps always sit in its own region;
+// they are cleared only when that capacity grows.
+
+namespace gsofa {
+
+int pack_matrix(const gsofa_matrix* A, PackedMatrix* M) {
+    const int rc = validate(A);
+    if (rc) return rc;
+    if (A->n < 1) return GSOFA_ERR_INVALID;
+    pack_rows(A, M->rp, M->ci);
+    M->n = A->n;
+    M->nnz = (int64_t)M->ci.size();
+    transpose_rows(M->n, M->rp.data(), M->ci.data(), M->rpT, M->ciT);
+    gsofa_options o;
+    gsofa_options_default(&o);
+    output_need(o, M->n, M->rp.data(), M->ci.data(), &M->need_fill, &M->need_L, &M->need_U);
+    return GSOFA_OK;
+}
+
+int64_t reach_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fills, uint64_t lu) {
+    return block_fixed_bytes(n, nnz, n_mats, nchunks, std::max<uint64_t>(fills, 1), (int64_t)lu) +
+           4 * (n + 1) + 4 * nnz + 16 * (int64_t)n_mats;
+}
+
+int64_t arena_held_bytes(const gsofa_plan* P) { return P ? P->held : 0; }
+
+int arena_create(const gsofa_options* opt, gsofa_plan** out) {
+    if (!out) return GSOFA_ERR_INVALID;
+    *out = nullptr;
+    gsofa_plan* P = new gsofa_plan();
+    if (opt) P->opt = *opt;
+    else gsofa_options_default(&P->opt);
+    P->opt.mode = GSOFA_MODE_REACH;
+    P->opt.n_virtual_gpu = 1;
+    P->opt.virtual_gpu_id = -1;
+    P->opt.flags &= ~(GSOFA_FLAG_SPILL | GSOFA_FLAG_GRAPH);
+    P->mode = GSOFA_MODE_REACH;
+    P->arena = true;
+    P->device = P->opt.device;
+    if (hipSetDevice(P->device) != hipSuccess || hipStreamCreate(&P->stream) != hipSuccess) {
+        (void)hipGetLastError();
+        delete P;
+        return GSOFA_ERR_HIP;
+    }
+    *out = P;
+    return GSOFA_OK;
+}
+
+int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, int64_t budget) {
+    if (!P || !P->arena || !mats || count < 1) return GSOFA_ERR_INVALID;
+    HIPCHK(hipSetDevice(P->device));
+    int64_t n = 0, nnz = 0;
+    uint32_t W = 1;
+    uint64_t nf = 0, nl = 0, nu = 0;
+    for (int32_t b = 0; b < count; b++) {
+        n += mats[b]->n;
+        nnz += mats[b]->nnz;
+        W = std::max<uint32_t>(W, (uint32_t)mats[b]->n);
+        nf += mats[b]->need_fill;
+        nl += mats[b]->need_L;
+        nu += mats[b]->need_U;
+    }
+    if (n > kMaxN || nnz >= (int64_t)INT32_MAX) return GSOFA_ERR_TOO_LARGE;
+    // block-diagonal batch on the host: matrix b's vertices follow matrix b-1's
+    P->n = (int32_t)n;
+    P->nnz = nnz;
+    P->n_mats = count;
+    P->h_mat_base.assign((size_t)count + 1, 0);
+    P->h_rp.assign((size_t)n + 1, 0);
+    P->h_ci.resize((size_t)nnz);
+    std::vector<int32_t>& rpT = P->h_rpT;
+    rpT.assign((size_t)n + 1, 0);
+    P->h_ciT.resize((size_t)nnz);
+    int64_t v = 0, k = 0;
+    for (int32_t b = 0; b < count; b++) {
+        const PackedMatrix& M = *mats[b];
+        P->h_mat_base[b] = (uint32_t)v;
+        for (int32_t i = 0; i <= M.n; i++) {
+            P->h_rp[(size_t)(v + i)] = (int32_t)(k + M.rp[(size_t)i]);
+            rpT[(size_t)(v + i)] = (int32_t)(k + M.rpT[(size_t)i]);
+        }
+        for (int64_t e = 0; e < M.nnz; e++) {
+            P->h_ci[(size_t)(k + e)] = M.ci[(size_t)e] + (int32_t)v;
+            P->h_ciT[(size_t)(k + e)] = M.ciT[(size_t)e] + (int32_t)v;
+        }
+        v += M.n;
+        k += M.nnz;
+    }
+    P->h_mat_base[count] = (uint32_t)v;
+    // chunks: every matrix's, deepest first, each followed by its A^T twin.
+    // Arenas keep 64 sources per workgroup, so the visited words keep one
+    // layout from batch to batch.
+    const int bits = 64;
+    P->S = (uint32_t)bits;
+    P->rbits = bits;
+    P->W = W;
+    build_chunks(P);
+    std::vector<BlkChunk> both;
+    both.reserve(2 * P->h_chunks.size());
+    for (const BlkChunk& ch : P->h_chunks) {
+        both.push_back(ch);
+        BlkChunk t = ch;
+        t.stride |= rch_pass_bit();
+        both.push_back(t);
+    }
+    P->h_chunks.swap(both);
+    P->share_chunk_off.assign(2, 0);
+    P->share_chunk_off[1] = (uint32_t)P->h_chunks.size();
+    const uint64_t nchunks = P->h_chunks.size();
+    // workgroups: what the budget leaves after the batch's fixed arrays
+    const int64_t fixed = reach_fixed_bytes(n, nnz, count, nchunks, nf, nl + nu);
+    uint32_t nb = 0;
+    gsofa_options o = P->opt;
+    o.mem_budget_bytes = 0;
+    int rc = reach_grid(o, bits, nchunks, W, budget - fixed, &nb);
+    if (rc) return rc;
+    gsofa_plan::Caps& c = P->cap;
+    const size_t rows = (size_t)n + 1;
+    // The arena keeps each buffer at the largest size seen.  If keeping them
+    // and growing the rest would exceed the budget, it starts over with
+    // exactly this batch's sizes (the only path that frees device memory).
+    {
+        auto mx = [](size_t a, uint64_t b) { return (int64_t)std::max<uint64_t>(a, std::max<uint64_t>(b, 1)); };
+        const int64_t keep = 40 * mx(c.rp, rows) + 8 * mx(c.ci, (uint64_t)nnz) + 8 * mx(c.fills, nf) +
+                             8 * mx(c.L, nl) + 8 * mx(c.U, nu) + 8 * mx(c.scan, scan_tmp_words(n) + 2 * (size_t)n + 8) +
+                             reach_bytes((uint32_t)std::max<size_t>(c.nb, nb), (uint32_t)std::max<size_t>(c.wg, W), 64);
+        if (keep > budget && P->held > 0) {
+            HIPCHK(hipStreamSynchronize(P->stream));
+            release_device(P);
+            P->cap = gsofa_plan::Caps();
+            P->sn_cap = 0;
+            P->held = 0;
+        }
+    }
+    int e = 0;
+    if (c.rp < rows) {
+        dfree(P->d_rp);
+        dfree(P->d_rpT);
+        dfree(P->d_cntL);
+        dfree(P->d_cntU);
+        dfree(P->d_Lrp);
+        dfree(P->d_Urp);
+        dfree(P->d_rowfill);
+        dfree(P->d_row_fk);
+        e |= dalloc(&P->d_rp, rows);
+        e |= dalloc(&P->d_rpT, rows);
+        e |= dalloc(&P->d_cntL, rows);
+        e |= dalloc(&P->d_cntU, rows);
+        e |= dalloc(&P->d_Lrp, rows);
+        e |= dalloc(&P->d_Urp, rows);
+        e |= dalloc(&P->d_rowfill, rows);
+        e |= dalloc(&P->d_row_fk, rows);
+        c.rp = rows;
+    }
+    const size_t ne = (size_t)std::max<int64_t>(nnz, 1);
+    if (c.ci < ne) {
+        dfree(P->d_ci);
+        dfree(P->d_ciT);
+        e |= dalloc(&P->d_ci, ne);
+        e |= dalloc(&P->d_ciT, ne);
+        c.ci = ne;
+    }
+    const size_t nfe = (size_t)std::max<uint64_t>(nf, 1);
+    if (c.fills < nfe) {
+        dfree(P->d_fills);
+        e |= dalloc(&P->d_fills, nfe);
+        c.fills = nfe;
+    }
+    const size_t nle = (size_t)std::max<uint64_t>(nl, 1), nue = (size_t)std::max<uint64_t>(nu, 1);
+    if (c.L < nle) {
+        dfree(P->d_Lcol);
+        dfree(P->d_Ltmp);
+        e |= dalloc(&P->d_Lcol, nle);
+        e |= dalloc(&P->d_Ltmp, nle);
+        c.L = nle;
+    }
+    if (c.U < nue) {
+        dfree(P->d_Ucol);
+        dfree(P->d_Utmp);
+        e |= dalloc(&P->d_Ucol, nue);
+        e |= dalloc(&P->d_Utmp, nue);
+        c.U = nue;
+    }
+    P->scan_words = scan_tmp_words(n) + 2 * (size_t)n + 8;
+    if (c.scan < P->scan_words) {
+        dfree(P->d_scan);
+        e |= dalloc(&P->d_scan, P->scan_words);
+        c.scan = P->scan_words;
+    }
+    if (c.mats < (size_t)count + 1) {
+        dfree(P->d_mat_base);
+        dfree(P->d_mat_te);
+        dfree(P->d_mat_reach);
+        e |= dalloc(&P->d_mat_base, (size_t)count + 1);
+        e |= dalloc(&P->d_mat_te, (size_t)count + 1);
+        e |= dalloc(&P->d_mat_reach, (size_t)count + 1);
+        c.mats = (size_t)count + 1;
+    }
+    if (c.chunks < nchunks) {
+        dfree(P->d_chunks);
+        e |= dalloc(&P->d_chunks, (size_t)nchunks);
+        c.chunks = nchunks;
+    }
+    if (!P->d_ctrl) e |= dalloc(&P->d_ctrl, 1);
+    if (e) return GSOFA_ERR_ALLOC;
+    // per-workgroup arrays: one stride (their capacity) for every batch,
+    // cleared when they grow
+    if (!P->d_vis || c.wg < W || c.nb < nb) {
+        dfree(P->d_vis);
+        dfree(P->d_rmask);
+        dfree(P->d_qgv);
+        dfree(P->d_bstate);
+        c.wg = std::max<size_t>(W, c.wg);
+        c.nb = std::max<size_t>(nb, c.nb);
+        if (reach_bytes((uint32_t)c.nb, (uint32_t)c.wg, 64) > budget - fixed) {  // exactly this batch's
+            c.wg = W;
+            c.nb = nb;
+        }
+        const size_t nbw = c.nb * c.wg;
+        e |= dalloc((uint4**)&P->d_vis, nbw);
+        e |= dalloc((uint64_t**)&P->d_rmask, 2 * nbw);
+        e |= dalloc(&P->d_qgv, 2 * nbw);
+        e |= dalloc(&P->d_bstate, c.nb);
+        if (e) return GSOFA_ERR_ALLOC;
+        HIPCHK(hipMemsetAsync(P->d_vis, 0, 16 * nbw, P->stream));
+        HIPCHK(hipMemsetAsync(P->d_rmask, 0, 16 * nbw, P->stream));
+        HIPCHK(hipMemsetAsync(P->d_bstate, 0, sizeof(uint32_t) * c.nb, P->stream));
+    }
+    P->held = (int64_t)(c.rp * (4 + 4 + 4 + 4 + 8 + 8 + 8 + 8) + c.ci * 8 + c.fills * 8 + c.L * 8 + c.U * 8 +
+                        c.scan * 8 + c.mats * 20 + c.chunks * sizeof(BlkChunk) + sizeof(Ctrl)) +
+              reach_bytes((uint32_t)c.nb, (uint32_t)c.wg, 64);
+    P->W = (uint32_t)c.wg;  // the kernels' per-workgroup stride
+    P->nblocks = nb;
+    P->C = (int32_t)(nb * P->S);
+    P->fill_cap = std::max<uint64_t>(nf, 1);
+    P->Lcap = (int64_t)c.L;
+    P->Ucap = (int64_t)c.U;
+    P->bounded = true;
+    P->totals_known = true;
+    P->span_cap = 0xFFFFFFFFu;
+    if (const char* sp = getenv("GSOFA_OFFSET_SPAN")) {
+        const long vsp = atol(sp);
+        if (vsp >= 1 && vsp < (long)0xFFFFFFFFu) P->span_cap = (uint32_t)vsp;
+    }
+    // uploads (the host copies stay in the plan until the next load)
+    HIPCHK(hipMemcpyAsync(P->d_rp, P->h_rp.data(), sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice,
+                          P->stream));
+    HIPCHK(hipMemcpyAsync(P->d_rpT, rpT.data(), sizeof(int32_t) * ((size_t)n + 1), hipMemcpyHostToDevice,
+                          P->stream));
+    if (nnz) {
+        HIPCHK(hipMemcpyAsync(P->d_ci, P->h_ci.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice,
+                              P->stream));
+        HIPCHK(hipMemcpyAsync(P->d_ciT, P->h_ciT.data(), sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice,
+                              P->stream));
+    }
+    HIPCHK(hipMemcpyAsync(P->d_mat_base, P->h_mat_base.data(), sizeof(uint32_t) * ((size_t)count + 1),
+                          hipMemcpyHostToDevice, P->stream));
+    HIPCHK(hipMemcpyAsync(P->d_chunks, P->h_chunks.data(), sizeof(BlkChunk) * nchunks, hipMemcpyHostToDevice,
+                          P->stream));
+    P->executed = false;
+    P->host_valid = false;
+    return GSOFA_OK;
+}
+
+// Matrix `index` of the last execute, in its own numbering, copied straight
+// from the device (its columns are shifted to local numbering there first).
+int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
+    if (!P || !P->arena || !r || index < 0 || index >= P->n_mats) return GSOFA_ERR_INVALID;
+    if (!P->executed) return GSOFA_ERR_STATE;
+    memset(r, 0, sizeof(*r));
+    HIPCHK(hipSetDevice(P->device));
+    const int64_t b0 = P->h_mat_base[(size_t)index], b1 = P->h_mat_base[(size_t)index + 1];
+    const int32_t nb = (int32_t)(b1 - b0);
+    r->n = nb;
+    r->ms = P->ms;
+    r->traverse_ms = P->traverse_ms;
+    r->chunk = P->C;
+    r->mode = GSOFA_MODE_REACH;
+    r->grid_blocks = (int32_t)P->nblocks;
+    r->queue_capacity = (int64_t)P->W;
+    r->passes = P->passes;
+    r->n_super = -1;
+    unsigned long long te = 0, reach = 0;
+    HIPCHK(hipMemcpyAsync(&te, P->d_mat_te + index, sizeof(te), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK(hipMemcpyAsync(&reach, P->d_mat_reach + index, sizeof(reach), hipMemcpyDeviceToHost, P->stream));
+    r->te_alg = te;
+    r->reach = reach;
+    r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
+    if (!r->row_fill) return GSOFA_ERR_ALLOC;
+    if (!P->structure) {  // counts only: the kernels' per-row tallies
+        HIPCHK(hipMemcpyAsync(r->row_fill, P->d_row_fk + b0, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        r->te_alg = te;
+        r->reach = reach;
+        for (int32_t i = 0; i < nb; i++) r->fill_count += (uint64_t)r->row_fill[i];
+        return GSOFA_OK;
+    }
+    r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * ((size_t)nb + 1));
+    r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * ((size_t)nb + 1));
+    if (!r->L_rowptr || !r->U_rowptr) {
+        gsofa_result_free(r);
+        return GSOFA_ERR_ALLOC;
+    }
+    HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp + b0, sizeof(int64_t) * ((size_t)nb + 1), hipMemcpyDeviceToHost,
+                          P->stream));
+    HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp + b0, sizeof(int64_t) * ((size_t)nb + 1), hipMemcpyDeviceToHost,
+                          P->stream));
+    HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill + b0, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost,
+                          P->stream));
+    HIPCHK(hipStreamSynchronize(P->stream));
+    r->te_alg = te;
+    r->reach = reach;
+    const int64_t l0 = r->L_rowptr[0], l1 = r->L_rowptr[nb], u0 = r->U_rowptr[0], u1 = r->U_rowptr[nb];
+    for (int32_t i = 0; i <= nb; i++) {
+        r->L_rowptr[i] -= l0;
+        r->U_rowptr[i] -= u0;
+    }
+    for (int32_t i = 0; i < nb; i++) r->fill_count += (uint64_t)r->row_fill[i];
+    r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
+    r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
+    if (!r->L_col || !r->U_col) {
+        gsofa_result_free(r);
+        return GSOFA_ERR_ALLOC;
+    }
+    if (b0) {  // global to the matrix's own column numbers
+        HIPCHK(launch_add(P->d_Lcol + l0, (uint64_t)(l1 - l0), -(int32_t)b0, P->stream));
+        HIPCHK(launch_add(P->d_Ucol + u0, (uint64_t)(u1 - u0), -(int32_t)b0, P->stream));
+    }
+    HIPCHK(copy_d2h(r->L_col, P->d_Lcol + l0, sizeof(int32_t) * (size_t)(l1 - l0), P->stream));
+    HIPCHK(copy_d2h(r->U_col, P->d_Ucol + u0, sizeof(int32_t) * (size_t)(u1 - u0), P->stream));
+    if (P->supernodes && nb > 0) {
+        int32_t s0 = 0, s1 = 0;
+        HIPCHK(hipMemcpyAsync(&s0, P->d_supno + b0, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(&s1, P->d_supno + b1 - 1, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        // supernodes never cross matrices: (i-1) is never in L_i at a boundary
+        r->n_super = s1 - s0 + 1;
+        r->super_start = (int32_t*)malloc(sizeof(int32_t) * ((size_t)r->n_super + 1));
+        r->supno = (int32_t*)malloc(sizeof(int32_t) * (size_t)nb);
+        if (!r->super_start || !r->supno) {
+            gsofa_result_free(r);
+            return GSOFA_ERR_ALLOC;
+        }
+        HIPCHK(hipMemcpyAsync(r->super_start, P->d_sstart + s0, sizeof(int32_t) * (size_t)r->n_super,
+                              hipMemcpyDeviceToHost, P->stream));
+        HIPCHK(hipMemcpyAsync(r->supno, P->d_supno + b0, sizeof(int32_t) * (size_t)nb, hipMemcpyDeviceToHost,
+                              P->stream));
+        HIPCHK(hipStreamSynchronize(P->stream));
+        for (int32_t s = 0; s < r->n_super; s++) r->super_start[s] -= (int32_t)b0;
+        r->super_start[r->n_super] = nb;
+        for (int32_t i = 0; i < nb; i++) r->supno[i] -= s0;
+    }
+    HIPCHK(hipStreamSynchronize(P->stream));
+    return GSOFA_OK;
+}
+
+}  // namespace gsofa

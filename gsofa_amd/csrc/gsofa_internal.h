@@ -8,6 +8,8 @@
 
 #include "gsofa.h"
 
+struct gsofa_plan;
+
 namespace gsofa {
 
 // Device control block.  Zeroed by the host before every traversal launch.
@@ -181,6 +183,30 @@ const void* traverse_kernel_ptr();
 size_t traverse_lds_bytes();
 
 hipError_t launch_fill(uint32_t* p, uint64_t count, uint32_t value, hipStream_t s);
+hipError_t launch_add(int32_t* p, uint64_t count, int32_t delta, hipStream_t s);  // p[i] += delta
+
+// A matrix prepared on the host for a reach-mode arena (the request server
+// packs each request when it is submitted, in the submitting thread): rows
+// of A and of A^T packed, sorted and duplicate-free, and the output bounds.
+struct PackedMatrix {
+    int32_t n = 0;
+    int64_t nnz = 0;
+    std::vector<int32_t> rp, ci, rpT, ciT;
+    uint64_t need_fill = 0, need_L = 0, need_U = 0;
+};
+int pack_matrix(const gsofa_matrix* A, PackedMatrix* M);
+// Arena: a reach-mode plan that takes one batch after another in its own
+// buffers (grown to the largest batch seen, never freed between batches).
+int arena_create(const gsofa_options* opt, gsofa_plan** out);
+// Load a batch (device bytes within `budget`); then gsofa_plan_execute.
+int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, int64_t budget);
+// After the execute: per-matrix results copied straight from the device.
+int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* out);
+int64_t arena_held_bytes(const gsofa_plan* P);
+// Device bytes a batch of reach-mode matrices needs apart from its traversal
+// workgroups, and per traversal workgroup.
+int64_t reach_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fills,
+                          uint64_t lu);
 struct FinalParams {
     int32_t n;
     const int32_t* rp;

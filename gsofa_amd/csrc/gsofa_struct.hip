@@ -25,6 +25,12 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
         p[i] = v;
 }
 
+__global__ void k_add_i32(int32_t* p, uint64_t count, int32_t d) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] += d;
+}
+
 
 __device__ __forceinline__ bool row_processed(int32_t s, int32_t n_gpu, int32_t gpu_id) {
     return gpu_id < 0 || (s % n_gpu) == gpu_id;
@@ -451,6 +457,12 @@ hipError_t supernode_sub_write(const int32_t* sstart, int32_t ns, const int64_t*
                                const int64_t* xsub, int32_t* sub, hipStream_t s) {
     if (ns <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sub_write, grid_for(ns, SB / 64), dim3(SB), 0, s, sstart, ns, Urp, Ucol, xsub, sub);
+    return hipGetLastError();
+}
+
+hipError_t launch_add(int32_t* p, uint64_t count, int32_t delta, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_add_i32, grid_for((int64_t)count), dim3(SB), 0, s, p, count, delta);
     return hipGetLastError();
 }
 
