@@ -13,7 +13,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <mutex>
+#include <string>
 #include <vector>
+
+#include <dlfcn.h>
 
 #include "gsofa.h"
 #include "gsofa_internal.h"
@@ -701,6 +704,105 @@ int block_grid(const gsofa_options& o, int S, uint64_t nchunks, uint32_t W, int6
     return GSOFA_OK;
 }
 
+// ---- tuned knobs ---------------------------------------------------------------
+// The autotuner (gsofa_amd/tune.py) times the knobs that never change results
+// and writes its choice per workload signature to a text table, one line per
+// signature: "<signature> key=value ...".  The library reads the table named
+// by GSOFA_TUNE_CACHE (empty or "off": none), else tune_table.txt beside
+// libgsofa.so, once per process, and applies an entry's knobs wherever the
+// caller left them at their defaults.  The signature buckets the largest
+// matrix's vertex count and the mean row length to powers of two, and a
+// batch's matrix count to the next power of two.
+
+namespace {
+
+struct TuneEntry {
+    std::string sig;
+    int32_t mode = -1, block_slots = -1, chunk = -1, grid_blocks = -1, top_sources = -1, threshold = -1;
+    int64_t queue_capacity = -1;
+};
+
+std::vector<TuneEntry> load_tune_table() {
+    std::vector<TuneEntry> t;
+    std::string path;
+    if (const char* e = getenv("GSOFA_TUNE_CACHE")) {
+        path = e;
+        if (path.empty() || path == "off") return t;
+    } else {
+        Dl_info info;
+        if (!dladdr(reinterpret_cast<void*>(&load_tune_table), &info) || !info.dli_fname) return t;
+        path = info.dli_fname;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tune_table.txt";
+    }
+    FILE* f = fopen(path.c_str(), "r");
+    if (!f) return t;
+    char line[1024];
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        TuneEntry e;
+        char* save = nullptr;
+        char* tok = strtok_r(line, " \t\n", &save);
+        if (!tok) continue;
+        e.sig = tok;
+        while ((tok = strtok_r(nullptr, " \t\n", &save))) {
+            char* eq = strchr(tok, '=');
+            if (!eq) continue;
+            *eq = 0;
+            const long long v = atoll(eq + 1);
+            if (!strcmp(tok, "mode")) e.mode = (int32_t)v;
+            else if (!strcmp(tok, "block_slots")) e.block_slots = (int32_t)v;
+            else if (!strcmp(tok, "chunk")) e.chunk = (int32_t)v;
+            else if (!strcmp(tok, "grid_blocks")) e.grid_blocks = (int32_t)v;
+            else if (!strcmp(tok, "top_sources")) e.top_sources = (int32_t)v;
+            else if (!strcmp(tok, "wave_block_threshold")) e.threshold = (int32_t)v;
+            else if (!strcmp(tok, "queue_capacity")) e.queue_capacity = (int64_t)v;
+        }
+        t.push_back(e);
+    }
+    fclose(f);
+    return t;
+}
+
+const std::vector<TuneEntry>& tune_table() {
+    static const std::vector<TuneEntry> t = load_tune_table();
+    return t;
+}
+
+}  // namespace
+
+std::string tune_signature(int64_t n_max, int64_t nnz, int64_t n_total, int32_t n_mats) {
+    const double deg = (double)nnz / (double)std::max<int64_t>(n_total, 1);
+    const int ln = (int)std::ceil(std::log2((double)std::max<int64_t>(n_max, 2)));
+    const int ld = (int)std::lround(std::log2(std::max(deg, 1.0)));
+    const int b = n_mats <= 1 ? 1 : (1 << (int)std::ceil(std::log2((double)n_mats)));
+    char buf[64];
+    snprintf(buf, sizeof(buf), "n2^%d_deg2^%d_b%d", ln, ld, b);
+    return buf;
+}
+
+// The tuned knobs of the workload's signature, applied where `o` has its
+// defaults (mode AUTO, slots, chunk, grid, hybrid split, threshold and queue
+// capacity unset).  Returns true when an entry applied.
+bool apply_tuned(gsofa_options* o, int64_t n_max, int64_t nnz, int64_t n_total, int32_t n_mats) {
+    const std::string sig = tune_signature(n_max, nnz, n_total, n_mats);
+    for (const TuneEntry& e : tune_table()) {
+        if (e.sig != sig) continue;
+        if (o->mode != GSOFA_MODE_AUTO || o->block_slots || o->chunk || o->grid_blocks || o->top_sources ||
+            o->queue_capacity || (o->wave_block_threshold && o->wave_block_threshold != 4096))
+            return false;  // the caller chose knobs: they stand
+        if (e.mode >= 0) o->mode = e.mode;
+        if (e.block_slots >= 0) o->block_slots = e.block_slots;
+        if (e.chunk >= 0) o->chunk = e.chunk;
+        if (e.grid_blocks >= 0) o->grid_blocks = e.grid_blocks;
+        if (e.top_sources >= 0) o->top_sources = e.top_sources;
+        if (e.threshold > 0) o->wave_block_threshold = e.threshold;
+        if (e.queue_capacity >= 0) o->queue_capacity = e.queue_capacity;
+        return true;
+    }
+    return false;
+}
+
 // The mode AUTO resolves to: reach mode wherever it applies (one share, in
 // core, no hybrid split, sources per workgroup left open), else block mode
 // (grid mode for block-mode matrices past 32-bit column offsets is decided at
@@ -810,6 +912,7 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
     if (opt) o = *opt;
     else gsofa_options_default(&o);
     if (!A || A->n <= 0) return 0;
+    if (o.n_virtual_gpu <= 1 && !(o.flags & GSOFA_FLAG_SPILL)) apply_tuned(&o, A->n, A->nnz, A->n, 1);
     o.mode = resolve_mode(o);
     if (o.mode == GSOFA_MODE_GRID || (o.mode == GSOFA_MODE_AUTO && A->nnz >= kMaxBlockNnz)) {
         Sizing sz;
@@ -881,6 +984,8 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         if (P->opt.mode != GSOFA_MODE_REACH && P->opt.mode != GSOFA_MODE_AUTO) P->opt.mode = GSOFA_MODE_BLOCK;
     }
     if (P->opt.n_virtual_gpu < 1) P->opt.n_virtual_gpu = 1;
+    if (P->opt.n_virtual_gpu == 1 && !(P->opt.flags & GSOFA_FLAG_SPILL))
+        apply_tuned(&P->opt, P->W, A->nnz, A->n, n_mats);
     P->opt.mode = resolve_mode(P->opt);
     const bool reach = P->opt.mode == GSOFA_MODE_REACH;
     if (P->opt.virtual_gpu_id >= P->opt.n_virtual_gpu ||
@@ -2244,6 +2349,9 @@ int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, in
     const int64_t fixed = reach_fixed_bytes(n, nnz, count, nchunks, nf, nl + nu);
     uint32_t nb = 0;
     gsofa_options o = P->opt;
+    o.mem_budget_bytes = 0;
+    o.mode = GSOFA_MODE_AUTO;  // the tuned grid for this batch's signature (slots stay 64)
+    if (!apply_tuned(&o, W, nnz, n, count) || resolve_mode(o) != GSOFA_MODE_REACH) o = P->opt;
     o.mem_budget_bytes = 0;
     int rc = reach_grid(o, bits, nchunks, W, budget - fixed, &nb);
     if (rc) return rc;

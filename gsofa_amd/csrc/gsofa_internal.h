@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
 #include <vector>
 
 #include "gsofa.h"
@@ -141,6 +142,10 @@ uint32_t rch_pass_bit();
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s);
 int reach_resident(int device, uint32_t* out);
 int resolve_mode(const gsofa_options& o);
+// Tuned knobs (gsofa_amd/tune.py's table) for a workload: its signature, and
+// the entry applied to options left at their defaults (true if one applied).
+std::string tune_signature(int64_t n_max, int64_t nnz, int64_t n_total, int32_t n_mats);
+bool apply_tuned(gsofa_options* o, int64_t n_max, int64_t nnz, int64_t n_total, int32_t n_mats);
 int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits);
 int reach_grid(const gsofa_options& o, int bits, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out);
 void transpose_rows(int32_t n, const int32_t* rp, const int32_t* ci, std::vector<int32_t>& rpT,

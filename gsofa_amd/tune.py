@@ -1,141 +1,191 @@
 """Autotuner for the runtime knobs of the traversal.
 
-The knobs that remain at run time are the execution mode, the persistent grid
-size, the number of top sources run as whole-device groups and the sources
-per workgroup (block mode),
-and the concurrent-source chunk and wave/block expansion threshold (grid
-mode).  Results do not depend on any knob (the parity tests
-check this), only time does, so the tuner compares traversal times alone,
-with counts only (no L/U structure).
+Every knob here leaves the results unchanged (the parity tests check this);
+only time depends on them, so the tuner compares traversal times alone, with
+counts only (no L/U structure).  The knobs are the ones the reference fixed
+at compile time (with_space_optimization/symbfact.cuh:42,49,57,77,102):
 
-It times the whole workload by default.  A sample of the sources (one
-virtual-GPU share, with_space_optimization/symbfact.cuh:1084-1099) is
-available with shares > 1 but misleads across modes: block mode's time on a
-small sample is bounded by the latency of its deepest 32-source chunk rather
-than by the work (a 1/16 share of the 48^3 ND grid takes as long as the
-whole matrix in block mode), while grid mode scales with the work.
+* reach mode: sources per workgroup (64 or 32) and, for batches, the
+  concurrent-source chunk (64 .. 4096 sources, i.e. workgroups x slots);
+* block mode: sources per workgroup (32, 16, 8) and the hybrid split (top
+  sources run as whole-device groups);
+* grid mode: concurrent-source chunk, the wave/workgroup expansion threshold
+  and the frontier queue capacity.
 
-Choices are cached in a JSON file keyed by a workload signature: the vertex
-count and mean row length rounded to powers of two, and for batches the
-matrix count.
+It times the whole workload.  The choice is written, per workload signature,
+to a text table that the library itself reads (one line per signature,
+``<signature> key=value ...``): gsofa_plan_create, gsofa_batch_create and the
+request server's batches apply the entry's knobs wherever the caller left
+them at their defaults.  The library reads the table named by the
+GSOFA_TUNE_CACHE environment variable ("off": none), else tune_table.txt
+beside libgsofa.so.  The signature buckets the largest matrix's vertex count
+and the mean row length to powers of two, and a batch's matrix count to the
+next power of two (gsofa_api.hip, tune_signature).
 """
-import json
 import math
 import os
 import time
 
-DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "gsofa_amd", "tune.json")
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_TABLE = os.path.join(HERE, "tune_table.txt")
+MODES = {"auto": 0, "block": 1, "grid": 2, "hybrid": 3, "reach": 4}
+KEYS = ("mode", "block_slots", "chunk", "grid_blocks", "top_sources", "wave_block_threshold",
+        "queue_capacity")
 
-# runtime knobs of gsofa_options that never change results
-BLOCK_CANDIDATES = [
-    {"mode": "block"},
-    {"mode": "block", "grid_blocks": 384},
-] + [{"mode": "block", "top_sources": k} for k in (2048, 8192, 32768)] + [
-    {"mode": "block", "block_slots": s} for s in (16, 8)]
+REACH_CANDIDATES = [{"mode": "reach", "block_slots": 64}, {"mode": "reach", "block_slots": 32}]
+BLOCK_CANDIDATES = ([{"mode": "block", "block_slots": s} for s in (32, 16, 8)] +
+                    [{"mode": "block", "top_sources": k} for k in (2048, 8192)])
 GRID_CANDIDATES = [
-    {"mode": "grid", "chunk": c, "wave_block_threshold": th}
-    for c in (2048, 8192) for th in (256, 4096)
+    {"mode": "grid", "chunk": c, "wave_block_threshold": th, "queue_capacity": q}
+    for c in (2048, 8192) for th in (256, 4096) for q in (0, 1 << 22)
 ]
+# batches: the concurrent-source chunk, largest first (the sweep stops once a
+# candidate takes 4x the best so far: fewer sources in flight only get slower)
+BATCH_CHUNKS = (4096, 1024, 256, 64)
 
 
 def signature(A):
-    """Bucketed workload signature of a matrix or a list of matrices."""
+    """Bucketed workload signature of a matrix or a list of matrices (the
+    library's tune_signature)."""
     mats = A if isinstance(A, (list, tuple)) else [A]
     n = max(M.n for M in mats)
     nnz = sum(M.nnz for M in mats)
     tot = sum(M.n for M in mats)
     deg = nnz / max(tot, 1)
+    b = 1 if len(mats) <= 1 else 2 ** int(math.ceil(math.log2(len(mats))))
     return "n2^%d_deg2^%d_b%d" % (int(math.ceil(math.log2(max(n, 2)))),
-                                   int(round(math.log2(max(deg, 1.0)))),
-                                   len(mats) if len(mats) == 1 else 2 ** int(math.ceil(math.log2(len(mats)))))
+                                   int(round(math.log2(max(deg, 1.0)))), b)
 
 
-def _load(path):
+def read_table(path=DEFAULT_TABLE):
+    """{signature: options} of a tuned table ({} if absent)."""
+    out = {}
     try:
         with open(path) as f:
-            return json.load(f)
-    except (OSError, ValueError):
-        return {}
+            for line in f:
+                parts = line.split()
+                if not parts or parts[0].startswith("#"):
+                    continue
+                opts = {}
+                for kv in parts[1:]:
+                    k, _, v = kv.partition("=")
+                    if k in KEYS:
+                        opts[k] = int(v)
+                names = {v: k for k, v in MODES.items()}
+                if "mode" in opts:
+                    opts["mode"] = names.get(opts["mode"], opts["mode"])
+                out[parts[0]] = opts
+    except OSError:
+        pass
+    return out
 
 
-def _store(path, table):
-    os.makedirs(os.path.dirname(path), exist_ok=True)
+def write_table(table, path=DEFAULT_TABLE):
+    """Write {signature: options} in the library's format (atomically)."""
+    d = os.path.dirname(path)
+    if d:
+        os.makedirs(d, exist_ok=True)
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
-        json.dump(table, f, indent=1, sort_keys=True)
+        f.write("# gsofa_amd tuned knobs: <signature> key=value ... (written by gsofa_amd/tune.py)\n")
+        for sig in sorted(table):
+            opts = table[sig]
+            kv = []
+            for k in KEYS:
+                if k in opts:
+                    v = MODES[opts[k]] if k == "mode" else int(opts[k])
+                    kv.append("%s=%d" % (k, v))
+            f.write("%s %s\n" % (sig, " ".join(kv)))
     os.replace(tmp, path)
 
 
-def _time_candidate(A, opts, shares, reps):
+def _time_candidate(A, opts, reps):
     import gsofa_amd as g
-    kw = dict(opts)
     best = float("inf")
-    if isinstance(A, (list, tuple)):
-        with g.Plan(list(A), counts_only=True, **kw) as p:
-            for _ in range(reps):
-                p.execute()
-                best = min(best, p.fetch().traverse_ms)
-        return best
-    with g.Plan(A, counts_only=True, n_virtual_gpu=shares, virtual_gpu_id=0, **kw) as p:
+    with g.Plan(list(A) if isinstance(A, (list, tuple)) else A, counts_only=True, **opts) as p:
         for _ in range(reps):
             p.execute()
             best = min(best, p.fetch().traverse_ms)
     return best
 
 
-def tune(A, shares=1, reps=1, grid_mode=True, cache=DEFAULT_CACHE, log=None):
-    """Time every candidate on A (a matrix, or a list of matrices as one
-    batch; shares > 1: on one virtual-GPU share only) and return
-    (best_options, timings).  The choice is recorded in `cache` (None: not
-    recorded)."""
+def candidates(A, grid_mode=True):
     batch = isinstance(A, (list, tuple))
-    cands = list(BLOCK_CANDIDATES)
-    if grid_mode and not batch:
-        cands += GRID_CANDIDATES
-    shares = max(1, min(shares, (A.n if not batch else 1)))
+    if batch:
+        return list(REACH_CANDIDATES) + [{"mode": "block", "block_slots": s} for s in (16, 8)]
+    return list(REACH_CANDIDATES) + BLOCK_CANDIDATES + (GRID_CANDIDATES if grid_mode else [])
+
+
+def tune(A, reps=1, grid_mode=True, table=DEFAULT_TABLE, log=None):
+    """Time every candidate on A (a matrix, or a list of matrices as one
+    batch) and return (best_options, timings).  The choice is recorded in the
+    library's table at `table` (None: not recorded)."""
+    batch = isinstance(A, (list, tuple))
     timings = []
-    for c in cands:
+
+    def run(c):
         t0 = time.perf_counter()
-        ms = _time_candidate(A, c, shares, reps)
+        ms = _time_candidate(A, c, reps)
         timings.append({"options": c, "traverse_ms": ms})
         if log:
-            log("tune %s: %.2f ms (%.1f s)" % (json.dumps(c), ms, time.perf_counter() - t0))
+            log("tune %s: %.2f ms (%.1f s)" % (c, ms, time.perf_counter() - t0))
+        return ms
+
+    for c in candidates(A, grid_mode):
+        run(c)
+    if batch:  # the chunk sweep of the best reach width
+        bits = min((t for t in timings if t["options"]["mode"] == "reach"),
+                   key=lambda t: t["traverse_ms"])["options"]["block_slots"]
+        best_ms = min(t["traverse_ms"] for t in timings)
+        for chunk in BATCH_CHUNKS:
+            ms = run({"mode": "reach", "block_slots": bits, "chunk": chunk})
+            best_ms = min(best_ms, ms)
+            if ms > 4 * best_ms:
+                break
     best = min(timings, key=lambda r: r["traverse_ms"])
-    if cache:
-        table = _load(cache)
-        table[signature(A)] = {"options": best["options"], "traverse_ms": best["traverse_ms"],
-                               "shares": shares if not batch else 1}
-        _store(cache, table)
+    if table:
+        t = read_table(table)
+        t[signature(A)] = dict(best["options"])
+        write_table(t, table)
     return dict(best["options"]), timings
 
 
-def best_options(A, cache=DEFAULT_CACHE):
+def best_options(A, table=DEFAULT_TABLE):
     """Tuned options for A's signature, or {} (library defaults) if untuned."""
-    entry = _load(cache).get(signature(A)) if cache else None
-    return dict(entry["options"]) if entry else {}
+    return dict(read_table(table).get(signature(A), {}))
 
 
 def main(argv=None):
-    """python -m gsofa_amd.tune [--out FILE]: tune the standard workloads and
-    print one JSON record per workload."""
+    """python -m gsofa_amd.tune [--out FILE] [--table FILE]: tune the standard
+    workloads, record the choices in the library's table and print one JSON
+    record per workload."""
     import argparse
+    import json
     import sys
 
+    import numpy as np
+
+    import gsofa_amd as g
     import gsofa_amd.configs as cf
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None, help="also write the records to this JSON file")
-    ap.add_argument("--cache", default=DEFAULT_CACHE)
+    ap.add_argument("--table", default=DEFAULT_TABLE)
+    ap.add_argument("--only", default=None, help="comma-separated workload names")
     args = ap.parse_args(argv)
-    work = [("grid64 natural", cf.grid64_natural(), 1),
-            ("3D 48^3 ND", cf.grid3d_nd(), 1),
-            ("band n=659,033", cf.band(), 1),
-            ("batch of 256 ND 2D grids", cf.grid2d_nd_batch(), 1)]
+    d = np.load(os.path.join(os.path.dirname(HERE), "tests", "golden", "bcsstk18.npz"))
+    work = [("grid64", cf.grid64_natural()),
+            ("bcsstk18", g.CsrMatrix(int(d["n"]), d["colbeg"], d["colend"], d["csr"])),
+            ("grid3d_nd", cf.grid3d_nd()),
+            ("band", cf.band()),
+            ("grid2d_nd_batch", cf.grid2d_nd_batch())]
+    if args.only:
+        keep = set(args.only.split(","))
+        work = [w for w in work if w[0] in keep]
     out = []
-    for name, A, shares in work:
-        best, timings = tune(A, shares=shares, reps=1, cache=args.cache,
+    for name, A in work:
+        best, timings = tune(A, reps=1, table=args.table,
                              log=lambda *a: print(*a, file=sys.stderr, flush=True))
-        rec = {"workload": name, "signature": signature(A), "shares_sampled": shares,
-               "best": best, "timings": timings}
+        rec = {"workload": name, "signature": signature(A), "best": best, "timings": timings}
         print(json.dumps(rec), flush=True)
         out.append(rec)
     if args.out:

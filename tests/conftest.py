@@ -5,6 +5,9 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+# the tests check the library's own defaults; the tuned table it would apply
+# is exercised on its own (test_gpu_tuned.py, in a subprocess)
+os.environ.setdefault("GSOFA_TUNE_CACHE", "off")
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 

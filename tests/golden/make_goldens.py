@@ -1,6 +1,7 @@
 """Compute oracle goldens for the large workloads and write them as JSON.
 
-Run in this repository (CPU only):  python tests/golden/make_goldens.py [grid3d_nd] [band] [grid2d_nd_batch]
+Run in this repository (CPU only):
+    python tests/golden/make_goldens.py [grid3d_nd] [band] [grid2d_nd_batch] [stream]
 
 For each workload the CPU oracle (oracle/liboracle.so) produces the full
 structure; the fixture keeps its totals and SHA-256 digests of the sorted L/U
@@ -21,6 +22,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import gsofa_amd.configs as cf  # noqa: E402
 from oracle import oracle  # noqa: E402
 from digests import oracle_record  # noqa: E402
+
+
+STREAM_COUNT = 32
 
 
 def run(name, A, out):
@@ -47,9 +51,20 @@ def main(which):
         print("grid2d_nd_batch", len(recs), flush=True)
     if "band" in which:
         run("band", cf.band(), out)
+    if "stream" in which:
+        # the request stream's matrices (they do not depend on the rate)
+        recs = []
+        t = time.time()
+        for i, (_, kind, side, A) in enumerate(cf.stream(STREAM_COUNT, 500.0)):
+            r = oracle.symbfact(A.colbeg, A.colend, A.csr, A.n)
+            rec = oracle_record(A, r)
+            rec["kind"], rec["side"] = kind, side
+            recs.append(rec)
+            print("stream", i, kind, side, A.n, round(time.time() - t, 1), flush=True)
+        out["stream"] = recs
     with open(path, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
 
 
 if __name__ == "__main__":
-    main(sys.argv[1:] or ["grid3d_nd", "grid2d_nd_batch", "band"])
+    main(sys.argv[1:] or ["grid3d_nd", "grid2d_nd_batch", "band", "stream"])

@@ -320,9 +320,9 @@ def test_autotuner_choice_is_result_neutral(bcsstk18, tmp_path):
     import gsofa_amd as g
     from gsofa_amd import tune
     _say("case: autotune bcsstk18")
-    best, timings = tune.tune(bcsstk18, reps=1, cache=str(tmp_path / "t.json"))
-    assert len(timings) == len(tune.BLOCK_CANDIDATES) + len(tune.GRID_CANDIDATES)
-    assert tune.best_options(bcsstk18, cache=str(tmp_path / "t.json")) == best
+    best, timings = tune.tune(bcsstk18, reps=1, table=str(tmp_path / "t.txt"))
+    assert len(timings) == len(tune.candidates(bcsstk18))
+    assert tune.best_options(bcsstk18, table=str(tmp_path / "t.txt")) == best
     assert_same_structure(g.symbfact(bcsstk18, **best), oracle_of(bcsstk18))
 
 

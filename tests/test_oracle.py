@@ -382,17 +382,25 @@ def test_spill_refused_for_batches():
     assert g._options(spill=True).flags & g.GSOFA_FLAG_SPILL
 
 
-def test_tune_signature_and_cache(tmp_path):
-    """Autotuner host logic: signatures bucket sizes, the cache round-trips."""
+def test_tune_signature_and_table(tmp_path):
+    """Autotuner host logic: signatures bucket sizes, the library's text table
+    round-trips, every candidate is a symbfact option."""
     import gsofa_amd as g
+    import gsofa_amd.configs as cf
     from gsofa_amd import tune
     A = g.CsrMatrix.from_rows([[0, 1], [0, 1, 2], [1, 2]])
     B = g.CsrMatrix.from_rows([[0], [1], [2], [3]])
     assert tune.signature(A) == tune.signature(A)
     assert tune.signature([A, B]) != tune.signature(A)
-    path = str(tmp_path / "t.json")
-    assert tune.best_options(A, cache=path) == {}
-    tune._store(path, {tune.signature(A): {"options": {"mode": "block", "grid_blocks": 256}}})
-    assert tune.best_options(A, cache=path) == {"mode": "block", "grid_blocks": 256}
-    for c in tune.BLOCK_CANDIDATES + tune.GRID_CANDIDATES:  # every knob is a symbfact option
+    assert tune.signature(cf.grid64_natural()) == "n2^12_deg2^2_b1"
+    assert tune.signature(cf.grid2d_nd_batch()) == "n2^16_deg2^2_b256"
+    path = str(tmp_path / "t.txt")
+    assert tune.best_options(A, table=path) == {}
+    tune.write_table({tune.signature(A): {"mode": "reach", "block_slots": 32, "grid_blocks": 3},
+                      tune.signature(B): {"mode": "grid", "chunk": 2048, "queue_capacity": 1 << 22}}, path)
+    assert tune.best_options(A, table=path) == {"mode": "reach", "block_slots": 32, "grid_blocks": 3}
+    assert tune.best_options(B, table=path) == {"mode": "grid", "chunk": 2048, "queue_capacity": 1 << 22}
+    line = [x for x in open(path) if x.startswith(tune.signature(A))][0].split()
+    assert line[1:] == ["mode=4", "block_slots=32", "grid_blocks=3"]
+    for c in tune.REACH_CANDIDATES + tune.BLOCK_CANDIDATES + tune.GRID_CANDIDATES:
         g._options(**c)
