@@ -50,6 +50,59 @@ def algorithmic_bytes(n, reach, te, nnz_lu):
     return 8 * (n + reach) + 8 * te + 16 * reach + 4 * nnz_lu
 
 
+def host_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_threads():
+    """All the cores this process may use: the affinity set, capped by
+    OMP_NUM_THREADS where it is set (the GPU hosts give a GPU 16)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        n = min(n, int(os.environ.get("OMP_NUM_THREADS", n)))
+    except ValueError:
+        pass
+    return max(1, n)
+
+
+def cpu_full(threads):
+    """The oracle's timed traversal (counters only, best of 3) over whole
+    workloads: C1, BCSSTK18, C2 and the C4 batch."""
+    import numpy as np
+
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    from oracle import oracle
+    d = np.load(os.path.join(ROOT, "tests", "golden", "bcsstk18.npz"))
+    work = [("grid64 natural (C1)", [cf.grid64_natural()]),
+            ("BCSSTK18", [g.CsrMatrix(int(d["n"]), d["colbeg"], d["colend"], d["csr"])]),
+            ("3D 48^3 ND (C2)", [cf.grid3d_nd()]),
+            ("256 ND 2D grids (C4)", cf.grid2d_nd_batch())]
+    out = []
+    for name, mats in work:
+        best, te = float("inf"), 0
+        for _ in range(3 if sum(M.n for M in mats) < 200000 else 1):
+            sec, te = 0.0, 0
+            for M in mats:
+                s_, f_, t_, r_ = oracle.count_rows(M.colbeg, M.colend, M.csr, M.n, 0, M.n, threads)
+                sec += s_
+                te += t_
+            best = min(best, sec)
+        out.append({"workload": name, "seconds": round(best, 3), "te_alg": te,
+                    "gteps": round(te / best / 1e9, 3)})
+        log("cpu full:", json.dumps(out[-1]))
+    return out
+
+
 def cpu_baseline(A, target_s, threads):
     """Oracle on the top rows of the matrix (the heaviest), sized to ~target_s."""
     from oracle import oracle
@@ -63,9 +116,11 @@ def cpu_baseline(A, target_s, threads):
     if sec < target_s / 2 and rows < n:
         rows = min(n, int(rows * target_s / max(sec, 1e-3)))
         sec, f, te, r = oracle.count_rows(A.colbeg, A.colend, A.csr, n, n - rows, n, threads)
-    return {"value": te / sec / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": "oracle traversal (OpenMP, schedule dynamic) of rows [%d, %d) of the same "
-                      "matrix: %d rows, TE=%d edge checks in %.2f s" % (n - rows, n, rows, te, sec)}
+    rec = {"value": te / sec / 1e9, "unit": "GTEPS", "cores": threads, "kind": "port",
+           "sample": "oracle traversal (OpenMP, schedule dynamic) of rows [%d, %d) of the same "
+                     "matrix: %d rows, TE=%d edge checks in %.2f s" % (n - rows, n, rows, te, sec)}
+    rec.update(host_info())
+    return rec
 
 
 def copy_probe(nbytes=1 << 30, reps=3):
@@ -102,7 +157,13 @@ def batch_bench(steps, warmup):
             t0 = time.perf_counter()
             p.execute()
             times.append(time.perf_counter() - t0)
-        fills = [p.fetch_matrix(b).fill_count for b in range(len(mats))]
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from digests import lu_digest
+        recs = []
+        for b in range(len(mats)):
+            r = p.fetch_matrix(b)
+            recs.append((r.fill_count, lu_digest(r.L_rowptr, r.L_col, r.U_rowptr, r.U_col)))
+        fills = [f for f, _ in recs]
     batched_s = sum(times) / len(times)
     with g.Plan(mats, graph=True) as p:
         p.execute()  # runs, then captures the graph
@@ -126,7 +187,8 @@ def batch_bench(steps, warmup):
     if os.path.exists(gpath):
         gold = json.load(open(gpath)).get("grid2d_nd_batch")
         if gold:
-            gold_ok = [x["fill_count"] for x in gold] == fills == gfills
+            gold_ok = ([x["fill_count"] for x in gold] == fills == gfills and
+                       [x["lu"] for x in gold] == [d for _, d in recs])
     lat_ms = np.array(lat) * 1e3
     return {"workload": "256 ND 2D grids, side log-uniform [16,256], seed 4",
             "matrices": len(mats), "vertices": int(sum(A.n for A in mats)),
@@ -140,6 +202,33 @@ def batch_bench(steps, warmup):
             "one_by_one_latency_ms_p50": round(float(np.percentile(lat_ms, 50)), 3),
             "one_by_one_latency_ms_p99": round(float(np.percentile(lat_ms, 99)), 3),
             "matches_golden": gold_ok}
+
+
+def bcsstk18_bench(steps, warmup):
+    """BCSSTK18 (the reference's shipped input, n=11,948): one plan, `steps`
+    timed executes (traversal + L/U + supernodes, matrix resident), beside the
+    reference's logged times for the same input (35.63-42.12 ms per run,
+    with_space_optimization/space_optimization_log_1_264_GPU_warp_centric.dat:4,16,...)."""
+    import numpy as np
+
+    import gsofa_amd as g
+    d = np.load(os.path.join(ROOT, "tests", "golden", "bcsstk18.npz"))
+    A = g.CsrMatrix(int(d["n"]), d["colbeg"], d["colend"], d["csr"])
+    with g.Plan(A) as p:
+        for _ in range(max(warmup, 1)):
+            p.execute()
+        times = []
+        for _ in range(max(steps, 1)):
+            t0 = time.perf_counter()
+            p.execute()
+            times.append(time.perf_counter() - t0)
+        r = p.fetch()
+    ms = 1e3 * sum(times) / len(times)
+    return {"workload": "BCSSTK18 n=11948", "ms_per_execute": round(ms, 3),
+            "traverse_ms": round(r.traverse_ms, 3), "device_ms": round(r.ms, 3),
+            "gteps": round(r.te_alg / (ms / 1e3) / 1e9, 2), "fill_count": r.fill_count,
+            "matches_reference_log": r.fill_count == 980136, "mode": r.mode,
+            "reference_logged_ms": [35.63, 42.12]}
 
 
 def serve_bench(count, rates, inflight=4, max_batch=64):
@@ -230,9 +319,23 @@ def main():
                     help="arrival rates (req/s) of the request-stream leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of 1 GiB (HBM counter calibration)")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="run only the CPU oracle over whole workloads (C1, BCSSTK18, C2, C4)")
+    ap.add_argument("--only", choices=["batch", "serve", "bcsstk18"],
+                    help="run only this leg and print its record (profiling runs)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.cpu_full:
+        th = cpu_threads()
+        print(json.dumps({"cpu_full": cpu_full(th), "cores": th, **host_info()}), flush=True)
+        return
+    if args.only:
+        leg = {"batch": lambda: batch_bench(args.steps, args.warmup),
+               "serve": lambda: serve_bench(args.serve_count, [float(x) for x in args.serve_rates.split(",")]),
+               "bcsstk18": lambda: bcsstk18_bench(args.steps, args.warmup)}[args.only]()
+        print(json.dumps({args.only: leg}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported "
@@ -277,12 +380,29 @@ def main():
             traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
         except (OSError, ValueError):
             traffic = None
+    # the timed run's structure against the oracle's golden: totals, the
+    # digest of the sorted L/U rows, per-row fills and supernodes
     gold_ok = None
     gpath = os.path.join(ROOT, "tests", "golden", "goldens.json")
     if os.path.exists(gpath) and not args.n:
         gold = json.load(open(gpath)).get(args.workload)
-        if gold:
-            gold_ok = (gold["fill_count"] == res.fill_count and gold["te_alg"] == te)
+        if gold and res.L_rowptr is not None:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            from digests import array_digest, lu_digest
+            gold_ok = (gold["fill_count"] == res.fill_count and gold["te_alg"] == te and
+                       gold["reach"] == res.reach and
+                       gold["lu"] == lu_digest(res.L_rowptr, res.L_col, res.U_rowptr, res.U_col) and
+                       gold["row_fill"] == array_digest(res.row_fill, "<i8") and
+                       gold["super0"] == array_digest(res.super_start))
+    # the autotuner's entry for this workload (applied by the library)
+    from gsofa_amd import tune
+    tuned = {"signature": tune.signature(A), "entry": tune.best_options(A) or None,
+             "ran": {"mode": res.mode, "slots": res.chunk // max(res.grid_blocks, 1),
+                     "grid_blocks": res.grid_blocks}}
+    # measured HBM traffic of the traversal launch over its duration
+    hbm_frac = None
+    if traffic:
+        hbm_frac = round(traffic / (res.traverse_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4)
     out = {
         "metric": METRIC,
         "value": round(gteps, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
@@ -296,8 +416,11 @@ def main():
                    "hbm_gbs_algorithmic": round(balg / (ms_per_step / 1e3) / 1e9, 1)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+                     "traffic": traffic, "frac_hbm_counters": hbm_frac},
+        "tuned": tuned,
     }
+    out["bcsstk18"] = bcsstk18_bench(args.steps, args.warmup)
+    log("bcsstk18:", json.dumps(out["bcsstk18"]))
     if not args.no_batch:
         out["batch"] = batch_bench(args.steps, args.warmup)
         log("batch:", json.dumps(out["batch"]))
@@ -305,8 +428,7 @@ def main():
         out["serve"] = serve_bench(args.serve_count, [float(x) for x in args.serve_rates.split(",")])
         log("serve:", json.dumps(out["serve"]))
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        out["cpu_baseline"] = cpu_baseline(A, args.cpu_seconds, threads)
+        out["cpu_baseline"] = cpu_baseline(A, args.cpu_seconds, cpu_threads())
     print(json.dumps(out), flush=True)
 
 
