@@ -26,7 +26,7 @@ EXPORTS = [
     "gsofa_server_poll", "gsofa_server_wait", "gsofa_server_destroy",
     "gsofa_plan_fetch_supernodal", "gsofa_supernodal_free",
     "gsofa_write_binary", "gsofa_read_binary", "gsofa_map_binary", "gsofa_unmap_binary",
-    "gsofa_order_nd", "gsofa_permute", "gsofa_fill_bound",
+    "gsofa_order_nd", "gsofa_permute", "gsofa_fill_bound", "gsofa_counts_noalloc",
 ]
 GSOFA_PENDING = 1
 
@@ -125,6 +125,9 @@ def lib():
     L.gsofa_permute.restype = C.c_int
     L.gsofa_fill_bound.argtypes = [P(Matrix), C.c_void_p, P(C.c_uint64), P(C.c_uint64), P(C.c_uint64)]
     L.gsofa_fill_bound.restype = C.c_int
+    L.gsofa_counts_noalloc.argtypes = [C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]
+    L.gsofa_counts_noalloc.restype = C.c_int
     L.gsofa_write_binary.argtypes = [P(Matrix), C.c_char_p]
     L.gsofa_write_binary.restype = C.c_int
     L.gsofa_read_binary.argtypes = [C.c_char_p, P(Csr)]

@@ -189,6 +189,27 @@ int gsofa_fill_bound(const gsofa_matrix* A, int64_t* row_fill_bound, uint64_t* f
  * (with_space_optimization/symbfact.cuh:1553). */
 int gsofa_symbfact(const gsofa_matrix* A, const gsofa_options* opt, gsofa_result* out);
 
+/* Fill counts of one matrix with no allocation at all.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * counts-only block-mode execute, with every work array taken from a static
+ * device scratch of GSOFA_NOALLOC_SCRATCH_BYTES that each GPU the library
+ * runs on holds until the process ends.  All pointers are device memory:
+ *   d_rowptr  n+1 int32, d_col nnz int32: the rows of A packed in order, no
+ *             duplicate entries (any order within a row);
+ *   d_row_fill, d_row_te, d_row_reach  n int64 each: per-row fills, TE_alg
+ *             and |R_<(s)|, as gsofa_result's;
+ *   d_totals  4 uint64: fill_count, te_alg, reach, edge checks.
+ * The work is queued on `stream` (a hipStream_t; NULL = the default stream)
+ * and the call returns; wait on the stream before reading the outputs.
+ * GSOFA_ERR_BUDGET when one workgroup's arrays (72 bytes per vertex) and the
+ * work table do not fit the scratch (n above about 900,000).  Calls that
+ * overlap on one device share the scratch and race: serialise them. */
+#define GSOFA_NOALLOC_SCRATCH_BYTES (64u << 20)
+int gsofa_counts_noalloc(int32_t n, int64_t nnz, const int32_t* d_rowptr, const int32_t* d_col,
+                         int64_t* d_row_fill, int64_t* d_row_te, int64_t* d_row_reach, uint64_t* d_totals,
+                         int32_t device, void* stream);
+
 /* ---- device-resident plan ---------------------------------------------------
  * create: validate, upload the CSR and size the workspace (no traversal).
  * execute: traversal + L/U structure + supernodes on the device; the inputs
