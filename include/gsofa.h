@@ -121,7 +121,10 @@ typedef struct gsofa_options {
  *        struct(L) of A is struct(U) of A^T transposed).  Every vertex is
  *        expanded at most once per source.  Whole-matrix runs only (one share
  *        or every share in one execute, in core); per-row TE and |R| are not
- *        reported (row_te = row_reach = NULL), their totals are. */
+ *        reported (row_te = row_reach = NULL), their totals are.
+ * AUTO:  the tuned table's mode for the workload if it has one; else REACH
+ *        when n_virtual_gpu is 1, nothing spills and block_slots and
+ *        top_sources are 0; else BLOCK (GRID above 2^30 packed entries). */
 #define GSOFA_MODE_AUTO 0
 #define GSOFA_MODE_BLOCK 1
 #define GSOFA_MODE_GRID 2
@@ -155,7 +158,7 @@ typedef struct gsofa_result {
     double share_ms_min;  /* fastest / slowest share (reference log, symbfact.cuh:2530) */
     double share_ms_max;
     int32_t chunk;        /* concurrent sources actually used                          */
-    int32_t mode;         /* GSOFA_MODE_BLOCK, GSOFA_MODE_GRID or GSOFA_MODE_HYBRID    */
+    int32_t mode;         /* the mode that ran: GSOFA_MODE_REACH, _BLOCK, _GRID, _HYBRID */
     int32_t grid_blocks;  /* persistent grid actually used                             */
     int64_t queue_capacity;
     uint64_t levels;      /* frontier levels summed over groups                        */

@@ -2,7 +2,9 @@
 every 2^32 / n groups (with_space_optimization/symbfact.cuh:1928-1932,
 2388-2407).  At real sizes a plan reaches it only after many executes (about
 80 at n = 659,033), so GSOFA_OFFSET_SPAN, read at plan creation, lowers the
-span to a few groups here; every execute must stay bit-exact in each mode."""
+span to a few groups here; every execute must stay bit-exact in each mode.
+Reach mode has no labels, but its visited words carry a per-workgroup stamp
+that the same span bounds: the words are cleared when the stamps run out."""
 import sys
 
 import pytest
@@ -22,6 +24,8 @@ CASES = [
     ("block 8 slots", dict(mode="block", block_slots=8)),
     ("grid, chunk 1024", dict(mode="grid", chunk=1024)),
     ("hybrid, top 2048", dict(mode="block", top_sources=2048)),
+    ("reach 64 slots", dict(mode="reach", block_slots=64)),
+    ("reach 32 slots", dict(mode="reach", block_slots=32)),
 ]
 
 
