@@ -81,6 +81,7 @@ struct gsofa_plan {
     uint64_t* d_q1 = nullptr;
     uint64_t* d_fills = nullptr;
     uint64_t fill_cap = 0;
+    uint64_t fill_need = 0;  // bounded plans: the fills the row bounds allow
     Ctrl* d_ctrl = nullptr;
     // Output sized from the row bounds of gsofa_fill_bound: the fill stream
     // cannot overflow and the L/U columns exist before the traversal, so an
@@ -935,7 +936,7 @@ int64_t gsofa_plan_footprint(const gsofa_matrix* A, const gsofa_options* opt) {
         uint32_t nb = 0;
         if (reach_grid(o, bits, nchunks, (uint32_t)A->n, INT64_MAX / 4, &nb) != GSOFA_OK) return -1;
         return reach_bytes(nb, (uint32_t)A->n, bits) +
-               block_fixed_bytes(A->n, (int64_t)ci.size(), 1, nchunks, std::max<uint64_t>(nf, 1), (int64_t)(nl + nu)) +
+               block_fixed_bytes(A->n, (int64_t)ci.size(), 1, nchunks, rch_fill_room(nf), (int64_t)(nl + nu)) +
                4 * (int64_t)(A->n + 1) + 4 * (int64_t)ci.size();
     }
     const int32_t N = std::max(1, o.n_virtual_gpu);
@@ -1040,7 +1041,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     uint64_t need_fill = 0, need_L = 0, need_U = 0;
     if (P->opt.output_capacity <= 0 && n > 0) {
         output_need(P->opt, n, P->h_rp.data(), ci.data(), &need_fill, &need_L, &need_U);
-        const int64_t out_bytes = output_bytes(need_fill, need_L, need_U);
+        const int64_t out_bytes = output_bytes(reach ? rch_fill_room(need_fill) : need_fill, need_L, need_U);
         if (out_bytes <= avail / 2) {
             P->bounded = true;
             avail -= out_bytes;
@@ -1180,7 +1181,8 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     if ((P->opt.flags & GSOFA_FLAG_SPILL) && P->opt.output_capacity <= 0)  // one share's fills at a time
         P->fill_cap = std::max<uint64_t>(P->fill_cap / (uint64_t)std::max(1, P->opt.n_virtual_gpu), 1u << 20);
     if (P->bounded) {
-        P->fill_cap = std::max<uint64_t>(need_fill, 1);
+        P->fill_cap = P->mode == GSOFA_MODE_REACH ? rch_fill_room(need_fill) : std::max<uint64_t>(need_fill, 1);
+        P->fill_need = need_fill;
         P->Lcap = (int64_t)need_L;
         P->Ucap = (int64_t)need_U;
         e |= dalloc(&P->d_Lcol, (size_t)need_L);
@@ -1406,6 +1408,7 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         rp.q = P->d_qgv;
         rp.fills = P->d_fills;
         rp.fill_cap = P->structure ? P->fill_cap : 0;  // counts only: per-row tallies, no fill stream
+        rp.fill_blk = rch_fill_block(P->fill_need ? P->fill_need : P->fill_cap, P->nblocks);
         rp.ctrl = P->d_ctrl;
         rp.bstate = P->d_bstate;
         rp.row_fills = P->d_row_fk;
@@ -1417,6 +1420,8 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         if (!P->structure)  // both passes add their fills per row
             HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_row_fk), 2 * (uint64_t)n, 0u, P->stream));
         HIPCHK(launch_fill(&P->d_ctrl->src_next, 1, 0u, P->stream));
+        if (P->structure)  // reserved entries left unused must read as ~0
+            HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_fills), 2 * P->fill_cap, 0xFFFFFFFFu, P->stream));
         if (!P->capturing) HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
         if (rp.n_chunks > 0) HIPCHK(launch_reach(rp, P->rbits, P->stream));
         if (!P->capturing) HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
@@ -2256,7 +2261,7 @@ int pack_matrix(const gsofa_matrix* A, PackedMatrix* M) {
 }
 
 int64_t reach_fixed_bytes(int64_t n, int64_t nnz, int32_t n_mats, uint64_t nchunks, uint64_t fills, uint64_t lu) {
-    return block_fixed_bytes(n, nnz, n_mats, nchunks, std::max<uint64_t>(fills, 1), (int64_t)lu) +
+    return block_fixed_bytes(n, nnz, n_mats, nchunks, rch_fill_room(fills), (int64_t)lu) +
            4 * (n + 1) + 4 * nnz + 16 * (int64_t)n_mats;
 }
 
@@ -2362,7 +2367,7 @@ int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, in
     // exactly this batch's sizes (the only path that frees device memory).
     {
         auto mx = [](size_t a, uint64_t b) { return (int64_t)std::max<uint64_t>(a, std::max<uint64_t>(b, 1)); };
-        const int64_t keep = 40 * mx(c.rp, rows) + 8 * mx(c.ci, (uint64_t)nnz) + 8 * mx(c.fills, nf) +
+        const int64_t keep = 40 * mx(c.rp, rows) + 8 * mx(c.ci, (uint64_t)nnz) + 8 * mx(c.fills, rch_fill_room(nf)) +
                              8 * mx(c.L, nl) + 8 * mx(c.U, nu) + 8 * mx(c.scan, scan_tmp_words(n) + 2 * (size_t)n + 8) +
                              reach_bytes((uint32_t)std::max<size_t>(c.nb, nb), (uint32_t)std::max<size_t>(c.wg, W), 64);
         if (keep > budget && P->held > 0) {
@@ -2401,7 +2406,7 @@ int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, in
         e |= dalloc(&P->d_ciT, ne);
         c.ci = ne;
     }
-    const size_t nfe = (size_t)std::max<uint64_t>(nf, 1);
+    const size_t nfe = (size_t)rch_fill_room(nf);
     if (c.fills < nfe) {
         dfree(P->d_fills);
         e |= dalloc(&P->d_fills, nfe);
@@ -2473,7 +2478,8 @@ int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, in
     P->W = (uint32_t)c.wg;  // the kernels' per-workgroup stride
     P->nblocks = nb;
     P->C = (int32_t)(nb * P->S);
-    P->fill_cap = std::max<uint64_t>(nf, 1);
+    P->fill_cap = nfe;
+    P->fill_need = nf;
     P->Lcap = (int64_t)c.L;
     P->Ucap = (int64_t)c.U;
     P->bounded = true;

@@ -125,8 +125,9 @@ struct RchParams {
     void* vis;             // nblocks * W visited words (8 B at 32 slots, 16 B at 64)
     void* nmask;           // nblocks * 2 * W next-level slot masks (4 or 8 B)
     uint4* q;              // nblocks * 2 * W frontier entries (v, row begin, row end, -)
-    uint64_t* fills;
+    uint64_t* fills;       // entries a workgroup reserved but did not use hold ~0 (set before launch)
     uint64_t fill_cap;     // 0: count fills per row without writing the fill stream
+    uint32_t fill_blk;     // fill-stream entries a workgroup reserves at once (rch_fill_block)
     Ctrl* ctrl;
     uint32_t* bstate;      // per workgroup: stamp of its last mini-group
     unsigned long long* row_fills;  // counts only: fills per row, added by both passes (zeroed first)
@@ -139,6 +140,13 @@ bool rch_bits_valid(int bits);
 const void* rch_kernel_ptr(int bits);
 int64_t reach_bytes(uint32_t nb, uint32_t W, int bits);
 uint32_t rch_pass_bit();
+// Fill-stream room of a reach-mode run whose fills are at most `need`: its
+// workgroups reserve the stream in blocks and may leave part of each unused
+// (at most a sixteenth of a block, and the last block of each workgroup).
+// rch_fill_room holds for any grid of up to 1024 workgroups whose block size
+// comes from rch_fill_block(need, nblocks).
+uint64_t rch_fill_room(uint64_t need);
+uint32_t rch_fill_block(uint64_t need, uint32_t nblocks);
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s);
 int reach_resident(int device, uint32_t* out);
 int resolve_mode(const gsofa_options& o);

@@ -39,12 +39,18 @@
 //        from level 1 on: level 0 expands the sources themselves, whose
 //        neighbours are A's own entries).
 //   Targets are unique within a round, so the visited word, the next-level
-//   mask and the queue append need no global atomics.  The round's counters
+//   mask and the queue append need no global atomics.  Fills go to the
+//   fill stream through blocks that a workgroup reserves with one global
+//   atomic and hands out to its waves from LDS (the reserved entries it
+//   leaves unused hold ~0, which the structure pass skips), so waves of the
+//   whole grid do not queue on one counter.  The round's counters
 //   alternate by parity and the level's by level parity, so a round costs two
 //   barriers and a level no more.
 // * Sources of a mini-group form an arithmetic sequence first, first-stride,
 //   ... (the cyclic source map of symbfact.cuh:1084-1099 within a share), so
 //   "source j lies above m" is the prefix of slots j < ceil((first-m)/stride).
+#include <algorithm>
+
 #include "gsofa_device.h"
 #include "gsofa_internal.h"
 
@@ -104,6 +110,8 @@ struct RLds {
     uint32_t tcnt[2];                    // inserts of this round, by round parity
     uint32_t held[2];                    // some group holds unfolded edges, by round parity
     uint32_t chunk;
+    uint32_t fcur;                       // entries handed out of the current fill block
+    unsigned long long fbase;            // the block's first entry in the fill stream
 };
 
 // Hash slot of window-local target m, inserting it if new; FULLI when it is
@@ -259,9 +267,12 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
         S.hmask[i] = (M)0;
     }
     if (tid < 8) S.acc[tid] = 0;
+    const uint32_t fblk = p.fill_blk, fmax = p.fill_blk >> 4;  // block; largest batch served from it
     if (tid == 0) {
         S.qcnt[0] = S.qcnt[1] = 0;
         S.tcnt[0] = S.tcnt[1] = S.held[0] = S.held[1] = 0;
+        S.fcur = fblk;  // no block yet
+        S.fbase = 0;
     }
     uint32_t rb = 0;  // round parity
     for (;;) {
@@ -323,6 +334,12 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
             hv.pos = hv.end = 0u;
             hv.vm = (M)0;
             for (;;) {  // rounds, two barriers each
+                // a fresh fill block once the current one may not hold a
+                // batch; relax reads fbase and fcur only after the next barrier
+                if (tid == 0 && p.fill_cap != 0ull && S.fcur + fmax > fblk) {
+                    S.fbase = aadd64(&p.ctrl->fill_pos, fblk);
+                    S.fcur = 0;
+                }
                 rcombine<G, M>(S, hv, c, maskc, qc, size, cursor, rb, t);
                 if (hv.held && (lane % G) == 0u) atomicOr(&S.held[rb], 1u);
                 __syncthreads();
@@ -378,11 +395,18 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                     t.fills += nf;
                     if (__ballot(nf != 0u)) {
                         if (p.fill_cap != 0ull) {
-                            // (row << 32) | column, one reservation per wave
+                            // (row << 32) | column, one reservation per wave:
+                            // from the workgroup's block, or (a batch larger
+                            // than a sixteenth of a block, or one that no
+                            // longer fits it) straight from the stream
                             const uint32_t incl = wave_incl_scan(nf);
                             const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
                             uint64_t fp = 0;
-                            if (lane == 0) fp = aadd64(&p.ctrl->fill_pos, total);
+                            if (lane == 0) {
+                                const uint32_t old = total <= fmax ? atomicAdd(&S.fcur, total) : FULLI;
+                                fp = (old != FULLI && old + total <= fblk) ? S.fbase + old
+                                                                           : aadd64(&p.ctrl->fill_pos, total);
+                            }
                             fp = bcast0_64(fp) + (incl - nf);
                             for (M b = up; b; b &= b - (M)1, fp++) {
                                 const uint32_t src = c.first - O::ctz(b) * c.stride;
@@ -469,6 +493,21 @@ int64_t reach_bytes(uint32_t nb, uint32_t W, int bits) {
     const int64_t vw = bits == 64 ? 16 : 8, mw = bits / 8;
     return (int64_t)nb * ((vw + 2 * mw + 32) * (int64_t)W + 8);
 }
+
+// Every block a workgroup gives up held more than 15/16 of a block of fills:
+// a batch is served from a block only if it is at most a sixteenth of it,
+// and the block is given up once the rest may not hold such a batch.  With
+// its last block, a workgroup's fill-stream span is below 16/15 of its fills
+// plus one block; over the grid, below 16/15 need + nblocks * block.  The
+// block size keeps nblocks * block within need/8 + 2^20, which
+// rch_fill_room covers (the 64-entry floor keeps that for nblocks <= 16384).
+uint32_t rch_fill_block(uint64_t need, uint32_t nblocks) {
+    const uint64_t nb = std::max<uint32_t>(nblocks, 1u);
+    uint32_t blk = 1u << 16;
+    while (blk > 64u && nb * blk > need / 8 + (1ull << 20)) blk >>= 1;
+    return blk;
+}
+uint64_t rch_fill_room(uint64_t need) { return need + need / 15 + need / 8 + (1ull << 20) + 1; }
 
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s) {
     if (bits == 64) hipLaunchKernelGGL((k_reach_blk<8, uint64_t>), dim3(p.nblocks), dim3(RB), 0, s, p);

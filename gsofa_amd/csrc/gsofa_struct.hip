@@ -56,7 +56,8 @@ __global__ void k_count_rows(FinalParams f) {
 }
 
 // Fill-stream entries to read: the host's count, or the traversal's append
-// position left in device memory (capped at the stream's length).
+// position left in device memory (capped at the stream's length).  Entries
+// a workgroup reserved and left unused hold ~0 (row >= n) and are skipped.
 __device__ __forceinline__ uint64_t fills_in_stream(const FinalParams& f) {
     if (!f.n_fills_dev) return f.n_fills;
     const uint64_t pos = *f.n_fills_dev;
@@ -69,6 +70,7 @@ __global__ void k_count_fills(FinalParams f) {
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = f.fills[i];
         const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
+        if (row >= (uint32_t)f.n) continue;  // reserved, unused (~0)
         atomicAdd(col < row ? &f.cntL[row] : &f.cntU[row], 1u);
         atomicAdd((unsigned long long*)&f.row_fill[row], 1ull);
     }
@@ -104,6 +106,7 @@ __global__ void k_scatter_fills(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t e = f.fills[i];
         const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
+        if (row >= (uint32_t)f.n) continue;  // reserved, unused (~0)
         if (col < row) {
             const int64_t q = f.L_rowptr[row] + atomicAdd(&f.cntL[row], 1u);
             if (q < f.L_cap) tmpL[q] = (int32_t)col;
