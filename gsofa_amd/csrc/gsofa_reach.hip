@@ -69,6 +69,14 @@ constexpr int H = 1 << HB;
 constexpr int T = H / 2;              // inserts allowed per round (load <= 1/2 + RB/H)
 constexpr uint32_t FULLI = 0xFFFFFFFFu;
 constexpr uint32_t PASS_T = 0x80000000u;  // chunk stride bit: the A^T pass
+#ifndef GSOFA_RG
+#define GSOFA_RG 4
+#endif
+#ifndef GSOFA_RE
+#define GSOFA_RE 3
+#endif
+constexpr int RG = GSOFA_RG;  // lanes per frontier vertex
+constexpr int RE = GSOFA_RE;  // edges per lane and step
 static_assert(T + RB <= H, "insert overshoot stays within the entry list");
 
 template <class M>
@@ -159,8 +167,8 @@ __device__ __forceinline__ uint32_t grp_ballot(bool x, uint32_t grp) {
 
 // The frontier vertex a lane group holds across rounds (group-uniform: every
 // lane of the group loads the same queue entry and mask, so no broadcast is
-// needed): its slot mask, edge cursor, row end and the lanes of its current
-// edge group not yet folded.
+// needed): its slot mask, edge cursor, row end and, per lane, the edges of
+// its current step not yet folded.
 template <class M>
 struct RGrp {
     bool held;           // a vertex is in progress
@@ -181,18 +189,20 @@ struct RCur {
     bool tpass;
 };
 
-// Phase A: lane groups pop frontier vertices one at a time and fold each
-// edge group's targets into the table until the queue is exhausted or the
-// table is full.  Wave-uniform control flow; the groups hold independent
-// vertices.
-template <int G, class M>
+// Phase A: lane groups of G lanes pop frontier vertices one at a time; a
+// step of a group covers the next G*E edges of its vertex, lane j taking
+// edges j, j+G, ..., j+(E-1)G of the step (all E loads issued before the
+// first fold).  Each target is found or inserted in the table and the
+// vertex's mask folded into it, until the queue is exhausted or the table is
+// full.  Wave-uniform control flow; the groups hold independent vertices.
+template <int G, int E, class M>
 __device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const uint4* qc, uint32_t size,
                          uint32_t* cursor, uint32_t rb, RTally& t) {
     using O = MaskOps<M>;
     const uint32_t lane = __lane_id();
     const uint32_t grp = lane / G;
     const uint32_t j = lane % G;
-    constexpr uint32_t ALL = (1u << G) - 1u;
+    constexpr uint32_t ALL = (1u << E) - 1u;  // a lane's edges of one step
     for (;;) {
         const bool want = !hv.held && !hv.done && S.tcnt[rb] < (uint32_t)T;
         const uint64_t wl = __ballot(want && j == 0u);
@@ -225,29 +235,36 @@ __device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const
         }
         bool failed = false;
         if (hv.held) {
-            const uint32_t e = hv.pos + j;
-            if (((hv.pend >> j) & 1u) && e < hv.end) {
-                const uint32_t m = (uint32_t)c.ci[e] - c.wb;
-                const uint32_t hs = rtab_find<M>(S, &S.tcnt[rb], m);
-                if (hs == FULLI) failed = true;
-                else atomicOr(&S.hmask[hs], hv.vm);
+            uint32_t m[E];
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                const uint32_t e = hv.pos + j + (uint32_t)(G * k);
+                m[k] = FULLI;
+                if (((hv.pend >> k) & 1u) && e < hv.end) m[k] = (uint32_t)c.ci[e] - c.wb;
             }
+            uint32_t left = 0;
+#pragma unroll
+            for (int k = 0; k < E; k++) {
+                if (m[k] != FULLI) {
+                    const uint32_t hs = rtab_find<M>(S, &S.tcnt[rb], m[k]);
+                    if (hs == FULLI) left |= 1u << k;
+                    else atomicOr(&S.hmask[hs], hv.vm);
+                }
+            }
+            hv.pend = left;  // the unplaced edges stay for the next round
+            failed = left != 0u;
         }
         const uint32_t fm = grp_ballot<G>(failed, grp);
-        if (hv.held) {
-            if (fm) {
-                hv.pend = fm;  // keep the unplaced edges of this group for the next round
-            } else {
-                hv.pos += G;
-                hv.pend = ALL;
-                hv.held = hv.pos < hv.end;
-            }
+        if (hv.held && !fm) {
+            hv.pos += (uint32_t)(G * E);
+            hv.pend = ALL;
+            hv.held = hv.pos < hv.end;
         }
         if (__ballot(fm != 0u)) return;  // the table is full for everyone
     }
 }
 
-template <int G, class M>
+template <int G, int E, class M>
 __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
     using O = MaskOps<M>;
     using VW = typename O::V;
@@ -340,7 +357,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                     S.fbase = aadd64(&p.ctrl->fill_pos, fblk);
                     S.fcur = 0;
                 }
-                rcombine<G, M>(S, hv, c, maskc, qc, size, cursor, rb, t);
+                rcombine<G, E, M>(S, hv, c, maskc, qc, size, cursor, rb, t);
                 if (hv.held && (lane % G) == 0u) atomicOr(&S.held[rb], 1u);
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
@@ -483,8 +500,8 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
 int rch_block_size() { return RB; }
 bool rch_bits_valid(int bits) { return bits == 32 || bits == 64; }
 const void* rch_kernel_ptr(int bits) {
-    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<8, uint64_t>)
-                      : reinterpret_cast<const void*>(&k_reach_blk<8, uint32_t>);
+    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<RG, RE, uint64_t>)
+                      : reinterpret_cast<const void*>(&k_reach_blk<RG, RE, uint32_t>);
 }
 uint32_t rch_pass_bit() { return PASS_T; }
 // Per workgroup: visited words (8 or 16 B), two next-level masks, two
@@ -510,8 +527,8 @@ uint32_t rch_fill_block(uint64_t need, uint32_t nblocks) {
 uint64_t rch_fill_room(uint64_t need) { return need + need / 15 + need / 8 + (1ull << 20) + 1; }
 
 hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s) {
-    if (bits == 64) hipLaunchKernelGGL((k_reach_blk<8, uint64_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
-    else if (bits == 32) hipLaunchKernelGGL((k_reach_blk<8, uint32_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
+    if (bits == 64) hipLaunchKernelGGL((k_reach_blk<RG, RE, uint64_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
+    else if (bits == 32) hipLaunchKernelGGL((k_reach_blk<RG, RE, uint32_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
     else return hipErrorInvalidValue;
     return hipGetLastError();
 }
