@@ -60,12 +60,14 @@ namespace {
 
 using namespace dev;
 
-#ifndef GSOFA_RHB
-#define GSOFA_RHB 13
+#ifndef GSOFA_RH
+#define GSOFA_RH 11264
 #endif
-constexpr int RB = 1024;              // threads per workgroup
-constexpr int HB = GSOFA_RHB;         // log2 hash slots
-constexpr int H = 1 << HB;
+#ifndef GSOFA_RB
+#define GSOFA_RB 1024
+#endif
+constexpr int RB = GSOFA_RB;          // threads per workgroup
+constexpr int H = GSOFA_RH;           // hash slots
 constexpr int T = H / 2;              // inserts allowed per round (load <= 1/2 + RB/H)
 constexpr uint32_t FULLI = 0xFFFFFFFFu;
 constexpr uint32_t PASS_T = 0x80000000u;  // chunk stride bit: the A^T pass
@@ -127,7 +129,7 @@ struct RLds {
 template <class M>
 __device__ __forceinline__ uint32_t rtab_find(RLds<M>& S, uint32_t* tcnt, uint32_t m) {
     const uint32_t key = m + 1u;
-    uint32_t s = (m * 2654435761u) >> (32 - HB);
+    uint32_t s = (uint32_t)(((uint64_t)(m * 2654435761u) * (uint32_t)H) >> 32);
     {
         const uint32_t cur = S.hkey[s];  // usual case: present at its first slot
         if (cur == key) return s;
@@ -144,7 +146,7 @@ __device__ __forceinline__ uint32_t rtab_find(RLds<M>& S, uint32_t* tcnt, uint32
             }
             if (prev == key) return s;
         }
-        s = (s + 1u) & (uint32_t)(H - 1);
+        s = s + 1u == (uint32_t)H ? 0u : s + 1u;
     }
     return FULLI;
 }
@@ -209,13 +211,22 @@ __device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const
         uint32_t base = 0;
         if (lane == 0 && wl) base = atomicAdd(cursor, (uint32_t)__popcll(wl));
         base = bcast0(base);
+        uint32_t m[E];     // targets of the current step
+        bool fresh = false;  // m holds the first step of a vertex just popped
         if (want) {
             const uint32_t idx = base + (uint32_t)__popcll(wl & ((1ull << (G * grp)) - 1ull));
             if (idx >= size) {
                 hv.done = true;
             } else {
                 const uint4 e = qc[idx];
+                // the mask and the first step's edges load together
                 const M mk = maskc[e.x];
+#pragma unroll
+                for (int k = 0; k < E; k++) {
+                    const uint32_t ek = e.y + j + (uint32_t)(G * k);
+                    m[k] = ek < e.z ? (uint32_t)c.ci[ek] - c.wb : FULLI;
+                }
+                fresh = true;
                 hv.pos = e.y;
                 hv.end = e.z;
                 hv.pend = ALL;
@@ -235,12 +246,13 @@ __device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const
         }
         bool failed = false;
         if (hv.held) {
-            uint32_t m[E];
+            if (!fresh) {
 #pragma unroll
-            for (int k = 0; k < E; k++) {
-                const uint32_t e = hv.pos + j + (uint32_t)(G * k);
-                m[k] = FULLI;
-                if (((hv.pend >> k) & 1u) && e < hv.end) m[k] = (uint32_t)c.ci[e] - c.wb;
+                for (int k = 0; k < E; k++) {
+                    const uint32_t e = hv.pos + j + (uint32_t)(G * k);
+                    m[k] = FULLI;
+                    if (((hv.pend >> k) & 1u) && e < hv.end) m[k] = (uint32_t)c.ci[e] - c.wb;
+                }
             }
             uint32_t left = 0;
 #pragma unroll
