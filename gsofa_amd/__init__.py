@@ -190,7 +190,7 @@ def write_ascii(A, colbeg_path, colend_path, csr_path):
 def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
              max_supernode_size=0, chunk=0, wave_block_threshold=0, device=0, queue_capacity=0,
              output_capacity=0, counts_only=False, grid_blocks=0, mode=0, top_sources=0,
-             block_slots=0, spill=False, graph=False):
+             block_slots=0, spill=False, graph=False, workgroup_threads=0):
     o = _lib.Options()
     _lib.lib().gsofa_options_default(C.byref(o))
     o.n_virtual_gpu = n_virtual_gpu
@@ -209,6 +209,7 @@ def _options(n_virtual_gpu=1, virtual_gpu_id=-1, mem_gb=0.0, mem_budget_bytes=0,
     o.mode = {"auto": 0, "block": 1, "grid": 2, "reach": 4}.get(mode, mode)
     o.top_sources = top_sources
     o.block_slots = block_slots
+    o.workgroup_threads = workgroup_threads
     return o
 
 

@@ -48,7 +48,7 @@ class Options(C.Structure):
                 ("queue_capacity", C.c_int64), ("output_capacity", C.c_int64),
                 ("flags", C.c_int32), ("grid_blocks", C.c_int32), ("mode", C.c_int32),
                 ("top_sources", C.c_int32),
-                ("block_slots", C.c_int32)]
+                ("block_slots", C.c_int32), ("workgroup_threads", C.c_int32)]
 
 
 class Result(C.Structure):

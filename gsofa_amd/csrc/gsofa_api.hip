@@ -66,6 +66,7 @@ struct gsofa_plan {
     void* d_vis = nullptr;
     void* d_rmask = nullptr;
     int rbits = 0;
+    int rthreads = 1024;  // reach mode: threads per workgroup
     unsigned long long* d_mat_te = nullptr;
     unsigned long long* d_mat_reach = nullptr;
     std::vector<unsigned long long> h_mat_te, h_mat_reach;  // last fetch (batches)
@@ -720,6 +721,7 @@ namespace {
 struct TuneEntry {
     std::string sig;
     int32_t mode = -1, block_slots = -1, chunk = -1, grid_blocks = -1, top_sources = -1, threshold = -1;
+    int32_t threads = -1;
     int64_t queue_capacity = -1;
 };
 
@@ -758,6 +760,7 @@ std::vector<TuneEntry> load_tune_table() {
             else if (!strcmp(tok, "top_sources")) e.top_sources = (int32_t)v;
             else if (!strcmp(tok, "wave_block_threshold")) e.threshold = (int32_t)v;
             else if (!strcmp(tok, "queue_capacity")) e.queue_capacity = (int64_t)v;
+            else if (!strcmp(tok, "workgroup_threads")) e.threads = (int32_t)v;
         }
         t.push_back(e);
     }
@@ -790,7 +793,8 @@ bool apply_tuned(gsofa_options* o, int64_t n_max, int64_t nnz, int64_t n_total, 
     for (const TuneEntry& e : tune_table()) {
         if (e.sig != sig) continue;
         if (o->mode != GSOFA_MODE_AUTO || o->block_slots || o->chunk || o->grid_blocks || o->top_sources ||
-            o->queue_capacity || (o->wave_block_threshold && o->wave_block_threshold != 4096))
+            o->queue_capacity || o->workgroup_threads ||
+            (o->wave_block_threshold && o->wave_block_threshold != 4096))
             return false;  // the caller chose knobs: they stand
         if (e.mode >= 0) o->mode = e.mode;
         if (e.block_slots >= 0) o->block_slots = e.block_slots;
@@ -799,6 +803,7 @@ bool apply_tuned(gsofa_options* o, int64_t n_max, int64_t nnz, int64_t n_total, 
         if (e.top_sources >= 0) o->top_sources = e.top_sources;
         if (e.threshold > 0) o->wave_block_threshold = e.threshold;
         if (e.queue_capacity >= 0) o->queue_capacity = e.queue_capacity;
+        if (e.threads >= 0) o->workgroup_threads = e.threads;
         return true;
     }
     return false;
@@ -830,15 +835,18 @@ int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits
         return GSOFA_OK;
     }
     uint32_t nb = 0;
-    int rc = reach_resident(o.device, &nb);
+    int rc = reach_resident(o.device, reach_threads(o), &nb);
     if (rc) return rc;
     *bits = (nsrc + 63) / 64 >= nb ? 64 : 32;
     return GSOFA_OK;
 }
 
-int reach_resident(int device, uint32_t* out) {
+int reach_threads(const gsofa_options& o) { return o.workgroup_threads != 0 ? o.workgroup_threads : 1024; }
+
+int reach_resident(int device, int threads, uint32_t* out) {
     int occ = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, rch_kernel_ptr(64), rch_block_size(), 0) != hipSuccess ||
+    if (!rch_threads_valid(threads)) return GSOFA_ERR_INVALID;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, rch_kernel_ptr(64, threads), threads, 0) != hipSuccess ||
         occ <= 0 || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) {
         (void)hipGetLastError();
         return GSOFA_ERR_HIP;
@@ -851,7 +859,7 @@ int reach_resident(int device, uint32_t* out) {
 int reach_grid(const gsofa_options& o, int bits, uint64_t nchunks, uint32_t W, int64_t avail, uint32_t* nb_out) {
     if (o.mem_budget_bytes > 0) avail = std::min<int64_t>(avail, o.mem_budget_bytes);
     uint32_t nb = 0;
-    const int rc = reach_resident(o.device, &nb);
+    const int rc = reach_resident(o.device, reach_threads(o), &nb);
     if (rc) return rc;
     if (o.grid_blocks > 0) nb = std::min<uint32_t>(nb, (uint32_t)o.grid_blocks);
     if (o.chunk > 0) nb = std::min<uint32_t>(nb, (uint32_t)std::max(1, o.chunk / bits));
@@ -1092,6 +1100,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         rc = choose_bits(P->opt, n_mats, (uint64_t)n, &bits);
         P->S = (uint32_t)bits;
         P->rbits = bits;
+        P->rthreads = reach_threads(P->opt);
         if (!rc) {
             // the A pass's chunks, deepest first, each followed by the same
             // sources' chunk of the A^T pass: one launch runs both
@@ -1423,7 +1432,7 @@ static int run_share(gsofa_plan* P, int32_t gid, size_t si) {
         if (P->structure)  // reserved entries left unused must read as ~0
             HIPCHK(launch_fill(reinterpret_cast<uint32_t*>(P->d_fills), 2 * P->fill_cap, 0xFFFFFFFFu, P->stream));
         if (!P->capturing) HIPCHK(hipEventRecord(P->ev[2 + 2 * si], P->stream));
-        if (rp.n_chunks > 0) HIPCHK(launch_reach(rp, P->rbits, P->stream));
+        if (rp.n_chunks > 0) HIPCHK(launch_reach(rp, P->rbits, P->rthreads, P->stream));
         if (!P->capturing) HIPCHK(hipEventRecord(P->ev[3 + 2 * si], P->stream));
         return GSOFA_OK;
     }
@@ -2360,6 +2369,7 @@ int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, in
     o.mem_budget_bytes = 0;
     int rc = reach_grid(o, bits, nchunks, W, budget - fixed, &nb);
     if (rc) return rc;
+    P->rthreads = reach_threads(o);
     gsofa_plan::Caps& c = P->cap;
     const size_t rows = (size_t)n + 1;
     // The arena keeps each buffer at the largest size seen.  If keeping them

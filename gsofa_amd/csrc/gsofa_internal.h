@@ -135,9 +135,9 @@ struct RchParams {
     unsigned long long* mat_reach;
     uint32_t loops;        // stamps per workgroup before its visited words are cleared
 };
-int rch_block_size();
+bool rch_threads_valid(int threads);  // threads per reach workgroup: 1024 or 512
 bool rch_bits_valid(int bits);
-const void* rch_kernel_ptr(int bits);
+const void* rch_kernel_ptr(int bits, int threads);
 int64_t reach_bytes(uint32_t nb, uint32_t W, int bits);
 uint32_t rch_pass_bit();
 // Fill-stream room of a reach-mode run whose fills are at most `need`: its
@@ -147,8 +147,9 @@ uint32_t rch_pass_bit();
 // comes from rch_fill_block(need, nblocks).
 uint64_t rch_fill_room(uint64_t need);
 uint32_t rch_fill_block(uint64_t need, uint32_t nblocks);
-hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s);
-int reach_resident(int device, uint32_t* out);
+hipError_t launch_reach(const RchParams& p, int bits, int threads, hipStream_t s);
+int reach_resident(int device, int threads, uint32_t* out);
+int reach_threads(const gsofa_options& o);  // 0 -> 1024
 int resolve_mode(const gsofa_options& o);
 // Tuned knobs (gsofa_amd/tune.py's table) for a workload: its signature, and
 // the entry applied to options left at their defaults (true if one applied).

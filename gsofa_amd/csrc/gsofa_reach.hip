@@ -60,15 +60,15 @@ namespace {
 
 using namespace dev;
 
-#ifndef GSOFA_RH
-#define GSOFA_RH 11264
-#endif
-#ifndef GSOFA_RB
-#define GSOFA_RB 1024
-#endif
-constexpr int RB = GSOFA_RB;          // threads per workgroup
-constexpr int H = GSOFA_RH;           // hash slots
-constexpr int T = H / 2;              // inserts allowed per round (load <= 1/2 + RB/H)
+// Workgroup shapes (threads, hash slots of the target table; a round takes
+// at most half the slots in new targets, so the load stays below 1/2 +
+// threads/slots):
+//   wide: one 1024-thread workgroup per CU and the largest table that fits
+//         the CU's 160 KB of LDS at 64 slots: fewest rounds per level;
+//   pair: two 512-thread workgroups per CU with half the table each: one
+//         workgroup runs while the other waits at a barrier.
+constexpr int NT_WIDE = 1024, NH_WIDE = 11264;
+constexpr int NT_PAIR = 512, NH_PAIR = 5632;
 constexpr uint32_t FULLI = 0xFFFFFFFFu;
 constexpr uint32_t PASS_T = 0x80000000u;  // chunk stride bit: the A^T pass
 #ifndef GSOFA_RG
@@ -79,7 +79,6 @@ constexpr uint32_t PASS_T = 0x80000000u;  // chunk stride bit: the A^T pass
 #endif
 constexpr int RG = GSOFA_RG;  // lanes per frontier vertex
 constexpr int RE = GSOFA_RE;  // edges per lane and step
-static_assert(T + RB <= H, "insert overshoot stays within the entry list");
 
 template <class M>
 struct MaskOps;
@@ -106,7 +105,7 @@ struct MaskOps<uint64_t> {
     static __device__ __forceinline__ uint32_t ctz(uint64_t m) { return (uint32_t)__builtin_ctzll(m); }
 };
 
-template <class M>
+template <class M, int H>
 struct RLds {
     uint32_t hkey[H];                    // target + 1 (window-local); 0 = empty
     M hmask[H];                          // OR of the masks folded into the target this round
@@ -126,8 +125,9 @@ struct RLds {
 
 // Hash slot of window-local target m, inserting it if new; FULLI when it is
 // new and the round already holds T targets.  Linear probing.
-template <class M>
-__device__ __forceinline__ uint32_t rtab_find(RLds<M>& S, uint32_t* tcnt, uint32_t m) {
+template <class M, int H>
+__device__ __forceinline__ uint32_t rtab_find(RLds<M, H>& S, uint32_t* tcnt, uint32_t m) {
+    constexpr int T = H / 2;
     const uint32_t key = m + 1u;
     uint32_t s = (uint32_t)(((uint64_t)(m * 2654435761u) * (uint32_t)H) >> 32);
     {
@@ -197,10 +197,11 @@ struct RCur {
 // first fold).  Each target is found or inserted in the table and the
 // vertex's mask folded into it, until the queue is exhausted or the table is
 // full.  Wave-uniform control flow; the groups hold independent vertices.
-template <int G, int E, class M>
-__device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const uint4* qc, uint32_t size,
+template <int H, int G, int E, class M>
+__device__ void rcombine(RLds<M, H>& S, RGrp<M>& hv, const RCur& c, M* maskc, const uint4* qc, uint32_t size,
                          uint32_t* cursor, uint32_t rb, RTally& t) {
     using O = MaskOps<M>;
+    constexpr int T = H / 2;
     const uint32_t lane = __lane_id();
     const uint32_t grp = lane / G;
     const uint32_t j = lane % G;
@@ -258,7 +259,7 @@ __device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const
 #pragma unroll
             for (int k = 0; k < E; k++) {
                 if (m[k] != FULLI) {
-                    const uint32_t hs = rtab_find<M>(S, &S.tcnt[rb], m[k]);
+                    const uint32_t hs = rtab_find<M, H>(S, &S.tcnt[rb], m[k]);
                     if (hs == FULLI) left |= 1u << k;
                     else atomicOr(&S.hmask[hs], hv.vm);
                 }
@@ -276,11 +277,13 @@ __device__ void rcombine(RLds<M>& S, RGrp<M>& hv, const RCur& c, M* maskc, const
     }
 }
 
-template <int G, int E, class M>
+template <int RB, int H, int G, int E, class M>
 __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
+    static_assert(H / 2 + RB <= H, "insert overshoot stays within the entry list");
+    static_assert(H <= 65536, "entries are 16-bit slot numbers");
     using O = MaskOps<M>;
     using VW = typename O::V;
-    __shared__ RLds<M> S;
+    __shared__ RLds<M, H> S;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = __lane_id();
     const uint32_t w = tid / WAVE;
@@ -369,7 +372,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                     S.fbase = aadd64(&p.ctrl->fill_pos, fblk);
                     S.fcur = 0;
                 }
-                rcombine<G, E, M>(S, hv, c, maskc, qc, size, cursor, rb, t);
+                rcombine<H, G, E, M>(S, hv, c, maskc, qc, size, cursor, rb, t);
                 if (hv.held && (lane % G) == 0u) atomicOr(&S.held[rb], 1u);
                 __syncthreads();
                 // final for this round: the cursor moves only in phase A
@@ -509,11 +512,14 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
 
 }  // namespace
 
-int rch_block_size() { return RB; }
 bool rch_bits_valid(int bits) { return bits == 32 || bits == 64; }
-const void* rch_kernel_ptr(int bits) {
-    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<RG, RE, uint64_t>)
-                      : reinterpret_cast<const void*>(&k_reach_blk<RG, RE, uint32_t>);
+bool rch_threads_valid(int threads) { return threads == NT_WIDE || threads == NT_PAIR; }
+const void* rch_kernel_ptr(int bits, int threads) {
+    if (threads == NT_PAIR)
+        return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint64_t>)
+                          : reinterpret_cast<const void*>(&k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint32_t>);
+    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint64_t>)
+                      : reinterpret_cast<const void*>(&k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint32_t>);
 }
 uint32_t rch_pass_bit() { return PASS_T; }
 // Per workgroup: visited words (8 or 16 B), two next-level masks, two
@@ -538,10 +544,15 @@ uint32_t rch_fill_block(uint64_t need, uint32_t nblocks) {
 }
 uint64_t rch_fill_room(uint64_t need) { return need + need / 15 + need / 8 + (1ull << 20) + 1; }
 
-hipError_t launch_reach(const RchParams& p, int bits, hipStream_t s) {
-    if (bits == 64) hipLaunchKernelGGL((k_reach_blk<RG, RE, uint64_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
-    else if (bits == 32) hipLaunchKernelGGL((k_reach_blk<RG, RE, uint32_t>), dim3(p.nblocks), dim3(RB), 0, s, p);
-    else return hipErrorInvalidValue;
+hipError_t launch_reach(const RchParams& p, int bits, int threads, hipStream_t s) {
+    if (!rch_bits_valid(bits) || !rch_threads_valid(threads)) return hipErrorInvalidValue;
+    if (threads == NT_PAIR) {
+        if (bits == 64) hipLaunchKernelGGL((k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint64_t>), dim3(p.nblocks), dim3(NT_PAIR), 0, s, p);
+        else hipLaunchKernelGGL((k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint32_t>), dim3(p.nblocks), dim3(NT_PAIR), 0, s, p);
+    } else {
+        if (bits == 64) hipLaunchKernelGGL((k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint64_t>), dim3(p.nblocks), dim3(NT_WIDE), 0, s, p);
+        else hipLaunchKernelGGL((k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint32_t>), dim3(p.nblocks), dim3(NT_WIDE), 0, s, p);
+    }
     return hipGetLastError();
 }
 

@@ -359,7 +359,7 @@ int gsofa_server_create(const gsofa_server_options* opt, gsofa_server** out) {
         o.mem_budget_bytes = (int64_t)((double)freeb * 0.80);
     }
     uint32_t resident = 0;
-    if (hipSetDevice(o.device) != hipSuccess || gsofa::reach_resident(o.device, &resident) != GSOFA_OK) {
+    if (hipSetDevice(o.device) != hipSuccess || gsofa::reach_resident(o.device, 1024, &resident) != GSOFA_OK) {
         (void)hipGetLastError();
         return GSOFA_ERR_HIP;
     }

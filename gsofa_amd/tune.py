@@ -5,8 +5,10 @@ only time depends on them, so the tuner compares traversal times alone, with
 counts only (no L/U structure).  The knobs are the ones the reference fixed
 at compile time (with_space_optimization/symbfact.cuh:42,49,57,77,102):
 
-* reach mode: sources per workgroup (64 or 32) and, for batches, the
-  concurrent-source chunk (64 .. 4096 sources, i.e. workgroups x slots);
+* reach mode: sources per workgroup (64 or 32), threads per workgroup (1024,
+  one per CU with the largest target table, or 512, two per CU) and, for
+  batches, the concurrent-source chunk (64 .. 4096 sources, i.e. workgroups x
+  slots);
 * block mode: sources per workgroup (32, 16, 8) and the hybrid split (top
   sources run as whole-device groups);
 * grid mode: concurrent-source chunk, the wave/workgroup expansion threshold
@@ -30,9 +32,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_TABLE = os.path.join(HERE, "tune_table.txt")
 MODES = {"auto": 0, "block": 1, "grid": 2, "hybrid": 3, "reach": 4}
 KEYS = ("mode", "block_slots", "chunk", "grid_blocks", "top_sources", "wave_block_threshold",
-        "queue_capacity")
+        "queue_capacity", "workgroup_threads")
 
-REACH_CANDIDATES = [{"mode": "reach", "block_slots": 64}, {"mode": "reach", "block_slots": 32}]
+REACH_CANDIDATES = [{"mode": "reach", "block_slots": 64}, {"mode": "reach", "block_slots": 32},
+                    {"mode": "reach", "block_slots": 64, "workgroup_threads": 512},
+                    {"mode": "reach", "block_slots": 32, "workgroup_threads": 512}]
 BLOCK_CANDIDATES = ([{"mode": "block", "block_slots": s} for s in (32, 16, 8)] +
                     [{"mode": "block", "top_sources": k} for k in (2048, 8192)])
 GRID_CANDIDATES = [
@@ -133,12 +137,12 @@ def tune(A, reps=1, grid_mode=True, table=DEFAULT_TABLE, log=None):
 
     for c in candidates(A, grid_mode):
         run(c)
-    if batch:  # the chunk sweep of the best reach width
-        bits = min((t for t in timings if t["options"]["mode"] == "reach"),
-                   key=lambda t: t["traverse_ms"])["options"]["block_slots"]
+    if batch:  # the chunk sweep of the best reach shape
+        shape = dict(min((t for t in timings if t["options"]["mode"] == "reach"),
+                         key=lambda t: t["traverse_ms"])["options"])
         best_ms = min(t["traverse_ms"] for t in timings)
         for chunk in BATCH_CHUNKS:
-            ms = run({"mode": "reach", "block_slots": bits, "chunk": chunk})
+            ms = run(dict(shape, chunk=chunk))
             best_ms = min(best_ms, ms)
             if ms > 4 * best_ms:
                 break

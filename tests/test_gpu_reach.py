@@ -242,3 +242,56 @@ def test_reach_band_full_golden():
         assert rec[k] == gold[k], k
     assert r.n_super == gold["n_super0"]
     assert array_digest(r.super_start) == gold["super0"]
+
+
+@pytest.mark.parametrize("bits", BITS)
+def test_reach_pair_workgroups(bcsstk18, bits):
+    """Two 512-thread workgroups per CU (half the target table each, so more
+    rounds per level) give the same results as the 1024-thread shape."""
+    import gsofa_amd as g
+    rng = np.random.default_rng(440 + bits)
+    cases = [bcsstk18, g.gen_grid(20, 20, 20, nd_leaf=64, extra_prob=0.05, extra_dist=2, seed=9),
+             g.gen_band(6000, 256, 8.0, 3)]
+    cases += [random_matrix(rng, int(rng.integers(1, 300)), 0.02)[0] for _ in range(4)]
+    for A in cases:
+        o = oracle_of(A)
+        _say("case: reach 512-thread workgroups n", A.n, "bits", bits)
+        with g.Plan(A, mode="reach", block_slots=bits, workgroup_threads=512) as p:
+            for it in range(2):
+                p.execute()
+                assert_same_structure(p.fetch(), o)
+
+
+def test_reach_pair_batch_vs_oracle():
+    import gsofa_amd as g
+    rng = np.random.default_rng(441)
+    mats = [random_matrix(rng, int(rng.integers(1, 200)), float(rng.choice([0.01, 0.05])))[0]
+            for _ in range(30)]
+    _say("case: reach 512-thread workgroups, batch of", len(mats))
+    with g.Plan(mats, mode="reach", workgroup_threads=512) as p:
+        p.execute()
+        for b, A in enumerate(mats):
+            assert_same_structure(p.fetch_matrix(b), oracle_of(A))
+
+
+def test_reach_pair_full_goldens():
+    """C2 (48^3 ND) and the n=659,033 band with 512-thread workgroups."""
+    import gsofa_amd as g
+    import gsofa_amd.configs as cf
+    from digests import array_digest
+    for name, A in (("grid3d_nd", cf.grid3d_nd()), ("band", cf.band())):
+        gold = _golden(name)
+        _say("case: reach 512-thread workgroups, full", name)
+        r = g.symbfact(A, mode="reach", block_slots=64, workgroup_threads=512)
+        rec = record_of(r)
+        for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
+            assert rec[k] == gold[k], (name, k)
+        assert array_digest(r.super_start) == gold["super0"]
+
+
+def test_reach_workgroup_threads_checked(bcsstk18):
+    import gsofa_amd as g
+    for t in (256, 768, -4):
+        with pytest.raises(g.GsofaError) as ei:
+            g.symbfact(bcsstk18, mode="reach", workgroup_threads=t)
+        assert ei.value.code == -1, t
