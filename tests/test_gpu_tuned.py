@@ -45,6 +45,10 @@ with g.Server(max_inflight=1, max_batch=8) as srv:  # the server's batch takes t
         if info["batch_size"] == 3:  # all three in one batch: the batch's entry
             assert rs.grid_blocks == 2, (info, rs.grid_blocks)
         assert_same_structure(rs, oracle.symbfact(M.colbeg, M.colend, M.csr, M.n))
+G = g.gen_grid(20, 20, 20, nd_leaf=64, extra_prob=0.05, extra_dist=2, seed=9)
+r = g.symbfact(G)  # tuned: reach, 64 slots, 512-thread workgroups, 7 of them
+assert (r.mode, r.grid_blocks, r.chunk) == ("reach", 7, 448), (r.mode, r.grid_blocks, r.chunk)
+assert_same_structure(r, oracle.symbfact(G.colbeg, G.colend, G.csr, G.n))
 print("tuned ok", flush=True)
 """
 
@@ -58,9 +62,14 @@ def test_library_applies_tuned_table(tmp_path):
     d = np.load(os.path.join(ROOT, "tests", "golden", "bcsstk18.npz"))
     A = g.CsrMatrix(int(d["n"]), d["colbeg"], d["colend"], d["csr"])
     mats = [cf.batch_matrix(s, b) for b, s in enumerate([16, 20, 24])]
+    G = g.gen_grid(20, 20, 20, nd_leaf=64, extra_prob=0.05, extra_dist=2, seed=9)
     table = str(tmp_path / "tune_table.txt")
     tune.write_table({tune.signature(A): {"mode": "reach", "block_slots": 32, "grid_blocks": 5},
-                      tune.signature(mats): {"mode": "reach", "block_slots": 64, "chunk": 128}}, table)
+                      tune.signature(mats): {"mode": "reach", "block_slots": 64, "chunk": 128},
+                      tune.signature(G): {"mode": "reach", "block_slots": 64, "grid_blocks": 7,
+                                          "workgroup_threads": 512}}, table)
+    assert tune.read_table(table)[tune.signature(G)]["workgroup_threads"] == 512
+    assert tune.signature(G) not in (tune.signature(A), tune.signature(mats))
     env = dict(os.environ, GSOFA_TUNE_CACHE=table)
     print("case: tuned table in a subprocess", flush=True)
     p = subprocess.run([sys.executable, "-c", CHILD, ROOT], env=env, capture_output=True, text=True, timeout=180)
