@@ -36,7 +36,8 @@ __device__ __forceinline__ bool row_processed(int32_t s, int32_t n_gpu, int32_t 
     return gpu_id < 0 || (s % n_gpu) == gpu_id;
 }
 
-// Per processed row: A's strictly-lower / strictly-upper entry counts.
+// Per processed row: A's strictly-lower / strictly-upper entry counts, and
+// minus their sum in row_fill (k_row_fill adds the final counts: the fills).
 __global__ void k_count_rows(FinalParams f) {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < f.n;
          s += (int64_t)gridDim.x * blockDim.x) {
@@ -51,8 +52,14 @@ __global__ void k_count_rows(FinalParams f) {
         }
         f.cntL[s] = l;
         f.cntU[s] = u;
-        f.row_fill[s] = proc ? 0 : -1;
+        f.row_fill[s] = proc ? -(int64_t)(l + u) : -1;
     }
+}
+
+__global__ void k_row_fill(FinalParams f) {
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < f.n;
+         s += (int64_t)gridDim.x * blockDim.x)
+        if (row_processed((int32_t)s, f.n_gpu, f.gpu_id)) f.row_fill[s] += (int64_t)f.cntL[s] + f.cntU[s];
 }
 
 // Fill-stream entries to read: the host's count, or the traversal's append
@@ -72,7 +79,6 @@ __global__ void k_count_fills(FinalParams f) {
         const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
         if (row >= (uint32_t)f.n) continue;  // reserved, unused (~0)
         atomicAdd(col < row ? &f.cntL[row] : &f.cntU[row], 1u);
-        atomicAdd((unsigned long long*)&f.row_fill[row], 1ull);
     }
 }
 
@@ -494,6 +500,7 @@ hipError_t exclusive_scan_u32_to_i64(const uint32_t* in, int64_t* out, int64_t n
 hipError_t launch_count(const FinalParams& f, hipStream_t s) {
     hipLaunchKernelGGL(k_count_rows, grid_for(f.n), dim3(SB), 0, s, f);
     if (f.n_fills) hipLaunchKernelGGL(k_count_fills, grid_for((int64_t)f.n_fills), dim3(SB), 0, s, f);
+    hipLaunchKernelGGL(k_row_fill, grid_for(f.n), dim3(SB), 0, s, f);
     return hipGetLastError();
 }
 

@@ -78,7 +78,37 @@ class OracleResult:
         self.__dict__.update(kw)
 
 
+# ORACLE_PROCESS=1: symbfact runs in a separate CPU-only worker process
+# (oracle/worker.py, one, started on first use), so the checker's memory is
+# never shared with the process that drives the GPU.
+_proc = None
+
+
+def _in_worker(*args):
+    global _proc
+    import pickle
+    import sys
+    if _proc is None:
+        import atexit
+        _proc = subprocess.Popen([sys.executable, "-m", "oracle.worker"], stdin=subprocess.PIPE,
+                                 stdout=subprocess.PIPE, cwd=os.path.dirname(_HERE))
+        atexit.register(_proc.stdin.close)
+    pickle.dump(args, _proc.stdin, protocol=pickle.HIGHEST_PROTOCOL)
+    _proc.stdin.flush()
+    ok, res = pickle.load(_proc.stdout)
+    if not ok:
+        raise ValueError(res)
+    return res
+
+
 def symbfact(colbeg, colend, csr, n, n_virtual_gpu=1, virtual_gpu_id=-1, nthreads=0):
+    if os.environ.get("ORACLE_PROCESS") == "1":
+        return _in_worker(np.asarray(colbeg), np.asarray(colend), np.asarray(csr), int(n),
+                          n_virtual_gpu, virtual_gpu_id, nthreads)
+    return _symbfact(colbeg, colend, csr, n, n_virtual_gpu, virtual_gpu_id, nthreads)
+
+
+def _symbfact(colbeg, colend, csr, n, n_virtual_gpu=1, virtual_gpu_id=-1, nthreads=0):
     m, keep = _mat(colbeg, colend, csr, n)
     r = _Result()
     rc = lib().orc_symbfact(C.byref(m), n_virtual_gpu, virtual_gpu_id, nthreads, C.byref(r))

@@ -14,6 +14,10 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and libgsofa.so")
+    # GPU runs: the oracle computes in its own CPU-only process (oracle.py)
+    mark = config.getoption("-m") or ""
+    if "gpu" in mark and "not gpu" not in mark:
+        os.environ.setdefault("ORACLE_PROCESS", "1")
 
 
 @pytest.fixture(scope="session")
