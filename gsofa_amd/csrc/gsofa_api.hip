@@ -821,9 +821,11 @@ int resolve_mode(const gsofa_options& o) {
     return GSOFA_MODE_AUTO;
 }
 
-// Reach mode: sources per workgroup (32 or 64).  Unless the options fix it,
-// a single matrix takes 64 when its chunks still give every co-resident
-// workgroup one, else 32; batches take 64.
+// Reach mode: sources per workgroup (32, 64 or 128).  Unless the options (or
+// the tuned table) fix it, a single matrix takes 64 when its chunks still
+// give every co-resident workgroup one, else 32; batches take 64.  128 pays
+// where the sources' searches advance together (banded matrices) and costs
+// where they do not (3D nested dissection), so only tuning picks it.
 int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits) {
     if (o.block_slots != 0) {
         if (!rch_bits_valid(o.block_slots)) return GSOFA_ERR_INVALID;
@@ -1128,7 +1130,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
         }
         P->C = (int32_t)(P->nblocks * P->S);
         const size_t nbw = (size_t)P->nblocks * (size_t)P->W;
-        e |= dalloc((uint64_t**)&P->d_vis, nbw * (bits == 64 ? 2 : 1));
+        e |= dalloc((uint64_t**)&P->d_vis, nbw * (size_t)(rch_visit_bytes(bits) / 8));
         e |= dalloc((uint32_t**)&P->d_rmask, 2 * nbw * (size_t)(bits / 32));
         e |= dalloc(&P->d_qgv, 2 * nbw);
         e |= dalloc(&P->d_bstate, (size_t)P->nblocks);
@@ -1233,7 +1235,7 @@ static int plan_create_impl(const gsofa_matrix* A, const uint32_t* mat_base, int
     }
     if (P->mode == GSOFA_MODE_REACH) {
         const size_t nbw = (size_t)P->nblocks * (size_t)P->W;
-        const size_t vb = nbw * (P->rbits == 64 ? 16 : 8), mb = 2 * nbw * (size_t)(P->rbits / 8);
+        const size_t vb = nbw * (size_t)rch_visit_bytes(P->rbits), mb = 2 * nbw * (size_t)(P->rbits / 8);
         ok = ok && hipMemsetAsync(P->d_vis, 0, vb, P->stream) == hipSuccess;
         ok = ok && hipMemsetAsync(P->d_rmask, 0, mb, P->stream) == hipSuccess;
         ok = ok && hipMemsetAsync(P->d_bstate, 0, sizeof(uint32_t) * P->nblocks, P->stream) == hipSuccess;

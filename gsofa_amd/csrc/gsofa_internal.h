@@ -146,6 +146,7 @@ uint32_t rch_pass_bit();
 // rch_fill_room holds for any grid of up to 1024 workgroups whose block size
 // comes from rch_fill_block(need, nblocks).
 uint64_t rch_fill_room(uint64_t need);
+int64_t rch_visit_bytes(int bits);  // visited word of a reach workgroup: 8, 16 or 32 B
 uint32_t rch_fill_block(uint64_t need, uint32_t nblocks);
 hipError_t launch_reach(const RchParams& p, int bits, int threads, hipStream_t s);
 int reach_resident(int device, int threads, uint32_t* out);

@@ -104,6 +104,52 @@ struct MaskOps<uint64_t> {
     static __device__ __forceinline__ uint32_t popc(uint64_t m) { return (uint32_t)__popcll(m); }
     static __device__ __forceinline__ uint32_t ctz(uint64_t m) { return (uint32_t)__builtin_ctzll(m); }
 };
+typedef unsigned __int128 u128;
+struct __align__(16) V128 {  // visited word of 128 slots: mask halves, stamp (32 B)
+    uint64_t lo, hi;
+    uint32_t stamp, pad[3];
+};
+template <>
+struct MaskOps<u128> {
+    using V = V128;
+    static constexpr int BITS = 128;
+    static __device__ __forceinline__ u128 get(const V& e, uint32_t st) {
+        return e.stamp == st ? (((u128)e.hi << 64) | e.lo) : (u128)0;
+    }
+    static __device__ __forceinline__ V make(u128 m, uint32_t st) {
+        V v;
+        v.lo = (uint64_t)m;
+        v.hi = (uint64_t)(m >> 64);
+        v.stamp = st;
+        v.pad[0] = v.pad[1] = v.pad[2] = 0u;
+        return v;
+    }
+    static __device__ __forceinline__ uint32_t popc(u128 m) {
+        return (uint32_t)(__popcll((uint64_t)m) + __popcll((uint64_t)(m >> 64)));
+    }
+    static __device__ __forceinline__ uint32_t ctz(u128 m) {
+        const uint64_t lo = (uint64_t)m;
+        return lo ? (uint32_t)__builtin_ctzll(lo) : 64u + (uint32_t)__builtin_ctzll((uint64_t)(m >> 64));
+    }
+};
+
+// OR into an LDS mask (128 slots: per 64-bit half that has bits).
+__device__ __forceinline__ void mask_or(uint32_t* p, uint32_t v) { atomicOr(p, v); }
+__device__ __forceinline__ void mask_or(uint64_t* p, uint64_t v) {
+    atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+}
+__device__ __forceinline__ void mask_or(u128* p, u128 v) {
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    if ((uint64_t)v) atomicOr(q, (unsigned long long)(uint64_t)v);
+    if ((uint64_t)(v >> 64)) atomicOr(q + 1, (unsigned long long)(uint64_t)(v >> 64));
+}
+
+// Hash slots of the target table for a shape and mask width: the largest
+// table that fits the CU's LDS (4 B key, the mask, 2 B entry per slot).
+template <class M>
+constexpr int tab_slots(int threads) {
+    return sizeof(M) == 16 ? (threads == NT_WIDE ? 7168 : 3584) : (threads == NT_WIDE ? NH_WIDE : NH_PAIR);
+}
 
 template <class M, int H>
 struct RLds {
@@ -261,7 +307,7 @@ __device__ void rcombine(RLds<M, H>& S, RGrp<M>& hv, const RCur& c, M* maskc, co
                 if (m[k] != FULLI) {
                     const uint32_t hs = rtab_find<M, H>(S, &S.tcnt[rb], m[k]);
                     if (hs == FULLI) left |= 1u << k;
-                    else atomicOr(&S.hmask[hs], hv.vm);
+                    else mask_or(&S.hmask[hs], hv.vm);
                 }
             }
             hv.pend = left;  // the unplaced edges stay for the next round
@@ -512,20 +558,38 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
 
 }  // namespace
 
-bool rch_bits_valid(int bits) { return bits == 32 || bits == 64; }
+bool rch_bits_valid(int bits) { return bits == 32 || bits == 64 || bits == 128; }
 bool rch_threads_valid(int threads) { return threads == NT_WIDE || threads == NT_PAIR; }
+
+namespace {
+template <int NT, class M>
+const void* kptr() {
+    return reinterpret_cast<const void*>(&k_reach_blk<NT, tab_slots<M>(NT), RG, RE, M>);
+}
+template <int NT, class M>
+void klaunch(const RchParams& p, hipStream_t s) {
+    hipLaunchKernelGGL((k_reach_blk<NT, tab_slots<M>(NT), RG, RE, M>), dim3(p.nblocks), dim3(NT), 0, s, p);
+}
+template <int NT>
+const void* kptr_bits(int bits) {
+    return bits == 128 ? kptr<NT, u128>() : bits == 64 ? kptr<NT, uint64_t>() : kptr<NT, uint32_t>();
+}
+template <int NT>
+void klaunch_bits(const RchParams& p, int bits, hipStream_t s) {
+    if (bits == 128) klaunch<NT, u128>(p, s);
+    else if (bits == 64) klaunch<NT, uint64_t>(p, s);
+    else klaunch<NT, uint32_t>(p, s);
+}
+}  // namespace
+
 const void* rch_kernel_ptr(int bits, int threads) {
-    if (threads == NT_PAIR)
-        return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint64_t>)
-                          : reinterpret_cast<const void*>(&k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint32_t>);
-    return bits == 64 ? reinterpret_cast<const void*>(&k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint64_t>)
-                      : reinterpret_cast<const void*>(&k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint32_t>);
+    return threads == NT_PAIR ? kptr_bits<NT_PAIR>(bits) : kptr_bits<NT_WIDE>(bits);
 }
 uint32_t rch_pass_bit() { return PASS_T; }
-// Per workgroup: visited words (8 or 16 B), two next-level masks, two
+// Per workgroup: visited words (8, 16 or 32 B), two next-level masks, two
 // 16-byte frontier queues, per vertex of the window.
 int64_t reach_bytes(uint32_t nb, uint32_t W, int bits) {
-    const int64_t vw = bits == 64 ? 16 : 8, mw = bits / 8;
+    const int64_t vw = rch_visit_bytes(bits), mw = bits / 8;
     return (int64_t)nb * ((vw + 2 * mw + 32) * (int64_t)W + 8);
 }
 
@@ -542,17 +606,13 @@ uint32_t rch_fill_block(uint64_t need, uint32_t nblocks) {
     while (blk > 64u && nb * blk > need / 8 + (1ull << 20)) blk >>= 1;
     return blk;
 }
+int64_t rch_visit_bytes(int bits) { return bits == 128 ? 32 : bits == 64 ? 16 : 8; }
 uint64_t rch_fill_room(uint64_t need) { return need + need / 15 + need / 8 + (1ull << 20) + 1; }
 
 hipError_t launch_reach(const RchParams& p, int bits, int threads, hipStream_t s) {
     if (!rch_bits_valid(bits) || !rch_threads_valid(threads)) return hipErrorInvalidValue;
-    if (threads == NT_PAIR) {
-        if (bits == 64) hipLaunchKernelGGL((k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint64_t>), dim3(p.nblocks), dim3(NT_PAIR), 0, s, p);
-        else hipLaunchKernelGGL((k_reach_blk<NT_PAIR, NH_PAIR, RG, RE, uint32_t>), dim3(p.nblocks), dim3(NT_PAIR), 0, s, p);
-    } else {
-        if (bits == 64) hipLaunchKernelGGL((k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint64_t>), dim3(p.nblocks), dim3(NT_WIDE), 0, s, p);
-        else hipLaunchKernelGGL((k_reach_blk<NT_WIDE, NH_WIDE, RG, RE, uint32_t>), dim3(p.nblocks), dim3(NT_WIDE), 0, s, p);
-    }
+    if (threads == NT_PAIR) klaunch_bits<NT_PAIR>(p, bits, s);
+    else klaunch_bits<NT_WIDE>(p, bits, s);
     return hipGetLastError();
 }
 

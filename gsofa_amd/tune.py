@@ -5,7 +5,7 @@ only time depends on them, so the tuner compares traversal times alone, with
 counts only (no L/U structure).  The knobs are the ones the reference fixed
 at compile time (with_space_optimization/symbfact.cuh:42,49,57,77,102):
 
-* reach mode: sources per workgroup (64 or 32), threads per workgroup (1024,
+* reach mode: sources per workgroup (64, 32 or 128), threads per workgroup (1024,
   one per CU with the largest target table, or 512, two per CU) and, for
   batches, the concurrent-source chunk (64 .. 4096 sources, i.e. workgroups x
   slots);
@@ -14,7 +14,8 @@ at compile time (with_space_optimization/symbfact.cuh:42,49,57,77,102):
 * grid mode: concurrent-source chunk, the wave/workgroup expansion threshold
   and the frontier queue capacity.
 
-It times the whole workload.  The choice is written, per workload signature,
+It times the whole workload; the remaining candidates of a mode are skipped
+once that mode's first candidate took over 4x the best so far.  The choice is written, per workload signature,
 to a text table that the library itself reads (one line per signature,
 ``<signature> key=value ...``): gsofa_plan_create, gsofa_batch_create and the
 request server's batches apply the entry's knobs wherever the caller left
@@ -34,9 +35,8 @@ MODES = {"auto": 0, "block": 1, "grid": 2, "hybrid": 3, "reach": 4}
 KEYS = ("mode", "block_slots", "chunk", "grid_blocks", "top_sources", "wave_block_threshold",
         "queue_capacity", "workgroup_threads")
 
-REACH_CANDIDATES = [{"mode": "reach", "block_slots": 64}, {"mode": "reach", "block_slots": 32},
-                    {"mode": "reach", "block_slots": 64, "workgroup_threads": 512},
-                    {"mode": "reach", "block_slots": 32, "workgroup_threads": 512}]
+REACH_CANDIDATES = [dict({"mode": "reach", "block_slots": b}, **({"workgroup_threads": 512} if t == 512 else {}))
+                    for t in (1024, 512) for b in (64, 32, 128)]
 BLOCK_CANDIDATES = ([{"mode": "block", "block_slots": s} for s in (32, 16, 8)] +
                     [{"mode": "block", "top_sources": k} for k in (2048, 8192)])
 GRID_CANDIDATES = [
@@ -135,18 +135,25 @@ def tune(A, reps=1, grid_mode=True, table=DEFAULT_TABLE, log=None):
             log("tune %s: %.2f ms (%.1f s)" % (c, ms, time.perf_counter() - t0))
         return ms
 
+    slow = set()  # modes whose first candidate took over 4x the best so far
     for c in candidates(A, grid_mode):
-        run(c)
+        if c["mode"] in slow:
+            timings.append({"options": c, "skipped": True})
+            continue
+        first = not any(t["options"]["mode"] == c["mode"] for t in timings)
+        ms = run(c)
+        if first and ms > 4 * min(t["traverse_ms"] for t in timings if "traverse_ms" in t):
+            slow.add(c["mode"])
     if batch:  # the chunk sweep of the best reach shape
         shape = dict(min((t for t in timings if t["options"]["mode"] == "reach"),
                          key=lambda t: t["traverse_ms"])["options"])
-        best_ms = min(t["traverse_ms"] for t in timings)
+        best_ms = min(t["traverse_ms"] for t in timings if "traverse_ms" in t)
         for chunk in BATCH_CHUNKS:
             ms = run(dict(shape, chunk=chunk))
             best_ms = min(best_ms, ms)
             if ms > 4 * best_ms:
                 break
-    best = min(timings, key=lambda r: r["traverse_ms"])
+    best = min((t for t in timings if "traverse_ms" in t), key=lambda r: r["traverse_ms"])
     if table:
         t = read_table(table)
         t[signature(A)] = dict(best["options"])

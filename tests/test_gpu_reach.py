@@ -2,7 +2,7 @@
 the CPU oracle, which restates the reference's max-id label traversal.
 
 Bit-exact on sorted L/U rows, per-row fills, TE_alg, |R| and supernodes, at
-32 and 64 sources per workgroup; full-size workloads against the committed
+32, 64 and 128 sources per workgroup; full-size workloads against the committed
 goldens.  Reach mode reports no per-row TE / |R| (their totals are checked).
 """
 import json
@@ -18,7 +18,7 @@ from gpu_util import assert_same_structure, record_of
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(ROOT, "tests", "golden", "goldens.json")
-BITS = [32, 64]
+BITS = [32, 64, 128]
 
 
 def _say(*a):
@@ -274,15 +274,16 @@ def test_reach_pair_batch_vs_oracle():
             assert_same_structure(p.fetch_matrix(b), oracle_of(A))
 
 
-def test_reach_pair_full_goldens():
+@pytest.mark.parametrize("bits", [64, 128])
+def test_reach_pair_full_goldens(bits):
     """C2 (48^3 ND) and the n=659,033 band with 512-thread workgroups."""
     import gsofa_amd as g
     import gsofa_amd.configs as cf
     from digests import array_digest
     for name, A in (("grid3d_nd", cf.grid3d_nd()), ("band", cf.band())):
         gold = _golden(name)
-        _say("case: reach 512-thread workgroups, full", name)
-        r = g.symbfact(A, mode="reach", block_slots=64, workgroup_threads=512)
+        _say("case: reach 512-thread workgroups, full", name, "bits", bits)
+        r = g.symbfact(A, mode="reach", block_slots=bits, workgroup_threads=512)
         rec = record_of(r)
         for k in ("fill_count", "te_alg", "reach", "nnz_L", "nnz_U", "lu", "row_fill"):
             assert rec[k] == gold[k], (name, k)
