@@ -821,11 +821,12 @@ int resolve_mode(const gsofa_options& o) {
     return GSOFA_MODE_AUTO;
 }
 
-// Reach mode: sources per workgroup (32, 64 or 128).  Unless the options (or
-// the tuned table) fix it, a single matrix takes 64 when its chunks still
-// give every co-resident workgroup one, else 32; batches take 64.  128 pays
-// where the sources' searches advance together (banded matrices) and costs
-// where they do not (3D nested dissection), so only tuning picks it.
+// Reach mode: sources per workgroup (32, 64, 128 or 256).  Unless the
+// options (or the tuned table) fix it, a single matrix takes 64 when its
+// chunks still give every co-resident workgroup one, else 32; batches take
+// 64.  Wider groups pay where the sources' searches advance together (banded
+// matrices) and cost where they do not (3D nested dissection), so only
+// tuning picks them.
 int choose_bits(const gsofa_options& o, int32_t n_mats, uint64_t nsrc, int* bits) {
     if (o.block_slots != 0) {
         if (!rch_bits_valid(o.block_slots)) return GSOFA_ERR_INVALID;

@@ -82,8 +82,17 @@ constexpr int RE = GSOFA_RE;  // edges per lane and step
 
 template <class M>
 struct MaskOps;
+// Bit helpers shared by the scalar mask types.
+template <class M, int B>
+struct ScalarBits {
+    static __device__ __forceinline__ M bit(uint32_t j) { return (M)1 << j; }
+    static __device__ __forceinline__ M prefix(uint32_t c) {  // the low c bits
+        return c >= (uint32_t)B ? ~(M)0 : (((M)1 << c) - (M)1);
+    }
+    static __device__ __forceinline__ M low_clear(M b) { return b & (b - (M)1); }
+};
 template <>
-struct MaskOps<uint32_t> {
+struct MaskOps<uint32_t> : ScalarBits<uint32_t, 32> {
     using V = uint2;  // visited word: (mask, stamp)
     static constexpr int BITS = 32;
     static __device__ __forceinline__ uint32_t get(const V& e, uint32_t st) { return e.y == st ? e.x : 0u; }
@@ -92,7 +101,7 @@ struct MaskOps<uint32_t> {
     static __device__ __forceinline__ uint32_t ctz(uint32_t m) { return (uint32_t)__builtin_ctz(m); }
 };
 template <>
-struct MaskOps<uint64_t> {
+struct MaskOps<uint64_t> : ScalarBits<uint64_t, 64> {
     using V = uint4;  // visited word: (mask lo, mask hi, stamp, -)
     static constexpr int BITS = 64;
     static __device__ __forceinline__ uint64_t get(const V& e, uint32_t st) {
@@ -110,7 +119,7 @@ struct __align__(16) V128 {  // visited word of 128 slots: mask halves, stamp (3
     uint32_t stamp, pad[3];
 };
 template <>
-struct MaskOps<u128> {
+struct MaskOps<u128> : ScalarBits<u128, 128> {
     using V = V128;
     static constexpr int BITS = 128;
     static __device__ __forceinline__ u128 get(const V& e, uint32_t st) {
@@ -133,7 +142,84 @@ struct MaskOps<u128> {
     }
 };
 
-// OR into an LDS mask (128 slots: per 64-bit half that has bits).
+// 256 slots: four 64-bit words.
+struct M256 {
+    uint64_t w[4];
+    M256() = default;
+    __device__ __forceinline__ constexpr M256(int z) : w{(uint64_t)z, 0ull, 0ull, 0ull} {}
+    __device__ __forceinline__ explicit operator bool() const { return (w[0] | w[1] | w[2] | w[3]) != 0ull; }
+};
+__device__ __forceinline__ M256 operator&(const M256& a, const M256& b) {
+    M256 r;
+    for (int k = 0; k < 4; k++) r.w[k] = a.w[k] & b.w[k];
+    return r;
+}
+__device__ __forceinline__ M256 operator|(const M256& a, const M256& b) {
+    M256 r;
+    for (int k = 0; k < 4; k++) r.w[k] = a.w[k] | b.w[k];
+    return r;
+}
+__device__ __forceinline__ M256 operator~(const M256& a) {
+    M256 r;
+    for (int k = 0; k < 4; k++) r.w[k] = ~a.w[k];
+    return r;
+}
+__device__ __forceinline__ bool operator==(const M256& a, const M256& b) {
+    return ((a.w[0] ^ b.w[0]) | (a.w[1] ^ b.w[1]) | (a.w[2] ^ b.w[2]) | (a.w[3] ^ b.w[3])) == 0ull;
+}
+struct __align__(16) V256 {  // visited word of 256 slots: mask words, stamp (48 B)
+    uint64_t w[4];
+    uint32_t stamp, pad[3];
+};
+template <>
+struct MaskOps<M256> {
+    using V = V256;
+    static constexpr int BITS = 256;
+    static __device__ __forceinline__ M256 get(const V& e, uint32_t st) {
+        M256 r(0);
+        if (e.stamp == st)
+            for (int k = 0; k < 4; k++) r.w[k] = e.w[k];
+        return r;
+    }
+    static __device__ __forceinline__ V make(const M256& m, uint32_t st) {
+        V v;
+        for (int k = 0; k < 4; k++) v.w[k] = m.w[k];
+        v.stamp = st;
+        v.pad[0] = v.pad[1] = v.pad[2] = 0u;
+        return v;
+    }
+    static __device__ __forceinline__ uint32_t popc(const M256& m) {
+        return (uint32_t)(__popcll(m.w[0]) + __popcll(m.w[1]) + __popcll(m.w[2]) + __popcll(m.w[3]));
+    }
+    static __device__ __forceinline__ uint32_t ctz(const M256& m) {
+        for (int k = 0; k < 3; k++)
+            if (m.w[k]) return 64u * (uint32_t)k + (uint32_t)__builtin_ctzll(m.w[k]);
+        return 192u + (uint32_t)__builtin_ctzll(m.w[3]);
+    }
+    static __device__ __forceinline__ M256 bit(uint32_t j) {
+        M256 r(0);
+        r.w[j >> 6] = 1ull << (j & 63u);
+        return r;
+    }
+    static __device__ __forceinline__ M256 prefix(uint32_t c) {
+        M256 r;
+        for (int k = 0; k < 4; k++) {
+            const uint32_t lo = 64u * (uint32_t)k;
+            r.w[k] = c >= lo + 64u ? ~0ull : c <= lo ? 0ull : ((1ull << (c - lo)) - 1ull);
+        }
+        return r;
+    }
+    static __device__ __forceinline__ M256 low_clear(M256 b) {
+        for (int k = 0; k < 4; k++)
+            if (b.w[k]) {
+                b.w[k] &= b.w[k] - 1ull;
+                break;
+            }
+        return b;
+    }
+};
+
+// OR into an LDS mask (128 and 256 slots: per 64-bit word that has bits).
 __device__ __forceinline__ void mask_or(uint32_t* p, uint32_t v) { atomicOr(p, v); }
 __device__ __forceinline__ void mask_or(uint64_t* p, uint64_t v) {
     atomicOr(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
@@ -143,12 +229,19 @@ __device__ __forceinline__ void mask_or(u128* p, u128 v) {
     if ((uint64_t)v) atomicOr(q, (unsigned long long)(uint64_t)v);
     if ((uint64_t)(v >> 64)) atomicOr(q + 1, (unsigned long long)(uint64_t)(v >> 64));
 }
+__device__ __forceinline__ void mask_or(M256* p, const M256& v) {
+    unsigned long long* q = reinterpret_cast<unsigned long long*>(p);
+    for (int k = 0; k < 4; k++)
+        if (v.w[k]) atomicOr(q + k, (unsigned long long)v.w[k]);
+}
 
 // Hash slots of the target table for a shape and mask width: the largest
 // table that fits the CU's LDS (4 B key, the mask, 2 B entry per slot).
 template <class M>
 constexpr int tab_slots(int threads) {
-    return sizeof(M) == 16 ? (threads == NT_WIDE ? 7168 : 3584) : (threads == NT_WIDE ? NH_WIDE : NH_PAIR);
+    return sizeof(M) == 32   ? (threads == NT_WIDE ? 4224 : 2048)
+           : sizeof(M) == 16 ? (threads == NT_WIDE ? 7168 : 3584)
+                             : (threads == NT_WIDE ? NH_WIDE : NH_PAIR);
 }
 
 template <class M, int H>
@@ -204,7 +297,7 @@ __device__ __forceinline__ M above_mask(uint32_t mg, uint32_t first, uint32_t st
     const uint32_t d = first - mg;
     uint32_t c = stride == 1u ? d : (d + stride - 1u) / stride;
     c = c < count ? c : count;
-    return c >= (uint32_t)MaskOps<M>::BITS ? ~(M)0 : (((M)1 << c) - (M)1);
+    return MaskOps<M>::prefix(c);
 }
 
 // The calling lane group's bits of a wave ballot (bit k: lane k of the group).
@@ -384,7 +477,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
         if (tid < c.count) {
             const uint32_t sg = c.first - tid * c.stride;
             const uint32_t s = sg - c.wb;
-            const M bit = (M)1 << tid;
+            const M bit = O::bit(tid);
             vis[s] = O::make(bit, stamp);
             nmask0[s] = bit;
             const uint32_t r0 = (uint32_t)c.rp[sg], r1 = (uint32_t)c.rp[sg + 1];
@@ -486,7 +579,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                                                                            : aadd64(&p.ctrl->fill_pos, total);
                             }
                             fp = bcast0_64(fp) + (incl - nf);
-                            for (M b = up; b; b &= b - (M)1, fp++) {
+                            for (M b = up; b; b = O::low_clear(b), fp++) {
                                 const uint32_t src = c.first - O::ctz(b) * c.stride;
                                 const uint64_t pr = c.tpass ? (((uint64_t)mg << 32) | src)
                                                             : (((uint64_t)src << 32) | mg);
@@ -496,7 +589,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                             if (c.tpass) {
                                 aadd64(&p.row_fills[mg], nf);
                             } else {
-                                for (M b = up; b; b &= b - (M)1) atomicAdd(&S.s_fill[O::ctz(b)], 1ull);
+                                for (M b = up; b; b = O::low_clear(b)) atomicAdd(&S.s_fill[O::ctz(b)], 1ull);
                             }
                         }
                     }
@@ -558,7 +651,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
 
 }  // namespace
 
-bool rch_bits_valid(int bits) { return bits == 32 || bits == 64 || bits == 128; }
+bool rch_bits_valid(int bits) { return bits == 32 || bits == 64 || bits == 128 || bits == 256; }
 bool rch_threads_valid(int threads) { return threads == NT_WIDE || threads == NT_PAIR; }
 
 namespace {
@@ -572,11 +665,15 @@ void klaunch(const RchParams& p, hipStream_t s) {
 }
 template <int NT>
 const void* kptr_bits(int bits) {
-    return bits == 128 ? kptr<NT, u128>() : bits == 64 ? kptr<NT, uint64_t>() : kptr<NT, uint32_t>();
+    return bits == 256   ? kptr<NT, M256>()
+           : bits == 128 ? kptr<NT, u128>()
+           : bits == 64  ? kptr<NT, uint64_t>()
+                         : kptr<NT, uint32_t>();
 }
 template <int NT>
 void klaunch_bits(const RchParams& p, int bits, hipStream_t s) {
-    if (bits == 128) klaunch<NT, u128>(p, s);
+    if (bits == 256) klaunch<NT, M256>(p, s);
+    else if (bits == 128) klaunch<NT, u128>(p, s);
     else if (bits == 64) klaunch<NT, uint64_t>(p, s);
     else klaunch<NT, uint32_t>(p, s);
 }
@@ -606,7 +703,7 @@ uint32_t rch_fill_block(uint64_t need, uint32_t nblocks) {
     while (blk > 64u && nb * blk > need / 8 + (1ull << 20)) blk >>= 1;
     return blk;
 }
-int64_t rch_visit_bytes(int bits) { return bits == 128 ? 32 : bits == 64 ? 16 : 8; }
+int64_t rch_visit_bytes(int bits) { return bits == 256 ? 48 : bits == 128 ? 32 : bits == 64 ? 16 : 8; }
 uint64_t rch_fill_room(uint64_t need) { return need + need / 15 + need / 8 + (1ull << 20) + 1; }
 
 hipError_t launch_reach(const RchParams& p, int bits, int threads, hipStream_t s) {

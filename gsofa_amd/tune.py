@@ -5,7 +5,7 @@ only time depends on them, so the tuner compares traversal times alone, with
 counts only (no L/U structure).  The knobs are the ones the reference fixed
 at compile time (with_space_optimization/symbfact.cuh:42,49,57,77,102):
 
-* reach mode: sources per workgroup (64, 32 or 128), threads per workgroup (1024,
+* reach mode: sources per workgroup (64, 32, 128 or 256), threads per workgroup (1024,
   one per CU with the largest target table, or 512, two per CU) and, for
   batches, the concurrent-source chunk (64 .. 4096 sources, i.e. workgroups x
   slots);
@@ -36,7 +36,7 @@ KEYS = ("mode", "block_slots", "chunk", "grid_blocks", "top_sources", "wave_bloc
         "queue_capacity", "workgroup_threads")
 
 REACH_CANDIDATES = [dict({"mode": "reach", "block_slots": b}, **({"workgroup_threads": 512} if t == 512 else {}))
-                    for t in (1024, 512) for b in (64, 32, 128)]
+                    for t in (1024, 512) for b in (64, 32, 128, 256)]
 BLOCK_CANDIDATES = ([{"mode": "block", "block_slots": s} for s in (32, 16, 8)] +
                     [{"mode": "block", "top_sources": k} for k in (2048, 8192)])
 GRID_CANDIDATES = [

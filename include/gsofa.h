@@ -102,7 +102,7 @@ typedef struct gsofa_options {
                                   /* groups (GSOFA_MODE_HYBRID); 0 = none                   */
     int32_t block_slots;          /* block mode: sources per workgroup, 32, 16 or 8; 0 = 32 */
                                   /* (fewer slots, larger per-round target table).  Reach   */
-                                  /* mode: 32, 64 or 128 (the bits of a visited word);       */
+                                  /* mode: 32, 64, 128 or 256 (bits of a visited word);      */
                                   /* 0 = 64, or 32 when fewer chunks than workgroups        */
     int32_t workgroup_threads;    /* reach mode: threads per workgroup, 1024 (one per CU,   */
                                   /* the largest target table) or 512 (two per CU, half    */

@@ -2,7 +2,7 @@
 the CPU oracle, which restates the reference's max-id label traversal.
 
 Bit-exact on sorted L/U rows, per-row fills, TE_alg, |R| and supernodes, at
-32, 64 and 128 sources per workgroup; full-size workloads against the committed
+32, 64, 128 and 256 sources per workgroup; full-size workloads against the committed
 goldens.  Reach mode reports no per-row TE / |R| (their totals are checked).
 """
 import json
@@ -18,7 +18,7 @@ from gpu_util import assert_same_structure, record_of
 pytestmark = pytest.mark.gpu
 
 GOLD = os.path.join(ROOT, "tests", "golden", "goldens.json")
-BITS = [32, 64, 128]
+BITS = [32, 64, 128, 256]
 
 
 def _say(*a):
