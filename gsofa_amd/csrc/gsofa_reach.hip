@@ -16,23 +16,27 @@
 // vertex each time its label drops (2-10 times per pair on 3D
 // nested-dissection grids).
 //
-// Execution model (CDNA4, one 1024-thread workgroup per CU, persistent grid):
-// * A workgroup owns MB source slots (32 or 64: the bits of a mask word) and
-//   runs a mini-group of up to MB sources of one matrix, in one of the two
-//   passes, through all of its levels alone.  The next mini-group comes from
-//   one global counter over a work table that interleaves the passes' chunks,
-//   deepest first, so both passes share one launch and its tail.
+// Execution model (CDNA4, persistent grid of co-resident workgroups: one of
+// 1024 threads per CU with the largest target table, or two of 512 threads
+// with half the table each, so one runs while the other waits at a barrier):
+// * A workgroup owns MB source slots (32, 64, 128 or 256: the bits of a mask
+//   of one, two or four 64-bit words) and runs a mini-group of up to MB
+//   sources of one matrix, in one of the two passes, through all of its
+//   levels alone.  The next mini-group comes from one global counter over a
+//   work table that interleaves the passes' chunks, deepest first, so both
+//   passes share one launch and its tail.
 // * Per vertex, one word holds the MB slots' visited bits with the
 //   mini-group's stamp beside it; an older stamp reads as "nothing visited",
 //   so the array is never cleared between mini-groups (the reference's
 //   source-stamped markers, symbfact.cuh:2126, generalised to a bit per slot).
 // * A level runs in rounds of two phases separated by workgroup barriers:
-//     A. combine: lane groups of G = 8 lanes pop frontier vertices (all
-//        lanes of a group load the same queue entry and mask, so nothing is
-//        broadcast between lanes); lane j of a group takes edge j of the current
-//        group of G edges, finds or inserts the target in an LDS hash table
-//        of the round's distinct targets and ORs the vertex's slot mask into
-//        the target's mask (ds_or);
+//     A. combine: lane groups of G lanes pop frontier vertices (all lanes of
+//        a group load the same queue entry and mask, so nothing is
+//        broadcast between lanes); a step covers G*E edges of the vertex,
+//        lane j taking edges j, j+G, ... with all E loads issued first; each
+//        target is found or inserted in an LDS hash table of the round's
+//        distinct targets and the vertex's slot mask ORed into the target's
+//        mask (ds_or);
 //     B. relax: one lane per table entry: new = mask & ~visited; the new bits
 //        of slots whose source lies above the target push it to the next
 //        frontier (its R_<(s) membership), the others are U entries (fills
