@@ -123,7 +123,7 @@ struct RchParams {
     const uint32_t* mat_base;
     uint32_t nblocks;
     void* vis;             // nblocks * W visited words (8 B at 32 slots, 16 B at 64)
-    void* nmask;           // nblocks * 2 * W next-level slot masks (4 or 8 B)
+    void* nmask;           // nblocks * 2 * W slot masks, this level's and the next's (slots/8 B each)
     uint4* q;              // nblocks * 2 * W frontier entries (v, row begin, row end, -)
     uint64_t* fills;       // entries a workgroup reserved but did not use hold ~0 (set before launch)
     uint64_t fill_cap;     // 0: count fills per row without writing the fill stream

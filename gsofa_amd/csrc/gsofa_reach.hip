@@ -94,6 +94,15 @@ struct ScalarBits {
         return c >= (uint32_t)B ? ~(M)0 : (((M)1 << c) - (M)1);
     }
     static __device__ __forceinline__ M low_clear(M b) { return b & (b - (M)1); }
+    template <class F>  // f(j) for every set bit j, in increasing order
+    static __device__ __forceinline__ void for_each_bit(M m, F f) {
+        if constexpr (B == 128) {
+            for (uint64_t b = (uint64_t)m; b; b &= b - 1ull) f((uint32_t)__builtin_ctzll(b));
+            for (uint64_t b = (uint64_t)(m >> 64); b; b &= b - 1ull) f(64u + (uint32_t)__builtin_ctzll(b));
+        } else {
+            for (M b = m; b; b &= b - (M)1) f((uint32_t)(B == 64 ? __builtin_ctzll(b) : __builtin_ctz(b)));
+        }
+    }
 };
 template <>
 struct MaskOps<uint32_t> : ScalarBits<uint32_t, 32> {
@@ -213,6 +222,11 @@ struct MaskOps<M256> {
         }
         return r;
     }
+    template <class F>
+    static __device__ __forceinline__ void for_each_bit(const M256& m, F f) {
+        for (int k = 0; k < 4; k++)
+            for (uint64_t b = m.w[k]; b; b &= b - 1ull) f(64u * (uint32_t)k + (uint32_t)__builtin_ctzll(b));
+    }
     static __device__ __forceinline__ M256 low_clear(M256 b) {
         for (int k = 0; k < 4; k++)
             if (b.w[k]) {
@@ -238,6 +252,17 @@ __device__ __forceinline__ void mask_or(M256* p, const M256& v) {
     for (int k = 0; k < 4; k++)
         if (v.w[k]) atomicOr(q + k, (unsigned long long)v.w[k]);
 }
+
+// One- and two-word masks (32, 64 slots) merge bits that reach a vertex still
+// queued for the current level into that level's mask, so it expands them in
+// the same pop: on 3D nested-dissection grids, where a vertex gets bits from
+// different sources at neighbouring levels, that halves the traversal.  The
+// two masks of a vertex (this level's, the next's) then sit side by side, so
+// relax reads both with one access.  Wider masks, whose sources' searches
+// mostly advance together (banded matrices), keep the two levels' masks in
+// separate arrays and do not merge: the extra reads cost more there.
+template <class M>
+constexpr bool merges() { return sizeof(M) <= 8; }
 
 // Hash slots of the target table for a shape and mask width: the largest
 // table that fits the CU's LDS (4 B key, the mask, 2 B entry per slot).
@@ -364,7 +389,7 @@ __device__ void rcombine(RLds<M, H>& S, RGrp<M>& hv, const RCur& c, M* maskc, co
             } else {
                 const uint4 e = qc[idx];
                 // the mask and the first step's edges load together
-                const M mk = maskc[e.x];
+                const M mk = maskc[(merges<M>() ? 2u : 1u) * e.x];
 #pragma unroll
                 for (int k = 0; k < E; k++) {
                     const uint32_t ek = e.y + j + (uint32_t)(G * k);
@@ -377,7 +402,7 @@ __device__ void rcombine(RLds<M, H>& S, RGrp<M>& hv, const RCur& c, M* maskc, co
                 hv.vm = mk;
                 hv.held = e.y < e.z;
                 if (j == 0u) {
-                    maskc[e.x] = (M)0;  // consumed; this buffer is refilled two levels later
+                    maskc[(merges<M>() ? 2u : 1u) * e.x] = (M)0;  // consumed; refilled two levels later
                     t.checks += (unsigned long long)(e.z - e.y) * O::popc(mk);
                     t.expansions++;
                 }
@@ -483,7 +508,7 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
             const uint32_t s = sg - c.wb;
             const M bit = O::bit(tid);
             vis[s] = O::make(bit, stamp);
-            nmask0[s] = bit;
+            nmask0[(merges<M>() ? 2u : 1u) * s] = bit;  // level 0 uses the first mask
             const uint32_t r0 = (uint32_t)c.rp[sg], r1 = (uint32_t)c.rp[sg + 1];
             q0[tid] = make_uint4(s, r0, r1, 0u);
             t.te += r1 - r0;
@@ -497,8 +522,11 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
             const uint32_t size = S.qcnt[cur];
             if (size == 0) break;
             const uint32_t nxt = cur ^ 1u;
-            M* maskc = nmask0 + (uint64_t)cur * W;
-            M* maskn = nmask0 + (uint64_t)nxt * W;
+            // this level's and the next level's masks: side by side per vertex
+            // (merging widths, index 2v) or two arrays (index v)
+            constexpr uint32_t MS = merges<M>() ? 2u : 1u;
+            M* maskc = nmask0 + (merges<M>() ? (uint64_t)cur : (uint64_t)cur * W);
+            M* maskn = nmask0 + (merges<M>() ? (uint64_t)nxt : (uint64_t)nxt * W);
             const uint4* qc = q0 + (uint64_t)cur * W;
             uint4* qn = q0 + (uint64_t)nxt * W;
             uint32_t* cursor = &S.cursor[lev & 1u];
@@ -541,7 +569,8 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                     bool enq = false;
                     if (ok) {
                         const VW vw = vis[m];
-                        const M mn = maskn[m];
+                        const M mc = merges<M>() ? maskc[MS * m] : (M)0;
+                        const M mn = maskn[MS * m];
                         r0 = (uint32_t)c.rp[mg];
                         r1 = (uint32_t)c.rp[mg + 1];
                         const M vv = O::get(vw, stamp);
@@ -551,8 +580,16 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                             const M bl = nw & above_mask<M>(mg, c.first, c.stride, c.count);
                             up = lev ? (nw & ~bl) : (M)0;  // level 0: A's own entries
                             if (bl) {
-                                maskn[m] = mn | bl;
-                                enq = mn == (M)0;
+                                if (merges<M>() && lev != 0u && mc) {
+                                    // still queued for this level and not yet
+                                    // popped: it expands these bits too, one
+                                    // expansion fewer (not at level 0, whose
+                                    // edges are A's own entries of the sources)
+                                    maskc[MS * m] = mc | bl;
+                                } else {
+                                    maskn[MS * m] = mn | bl;
+                                    enq = mn == (M)0;
+                                }
                                 const uint32_t nb = O::popc(bl);
                                 t.reach += nb;
                                 t.te += (unsigned long long)nb * (r1 - r0);
@@ -583,17 +620,18 @@ __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
                                                                            : aadd64(&p.ctrl->fill_pos, total);
                             }
                             fp = bcast0_64(fp) + (incl - nf);
-                            for (M b = up; b; b = O::low_clear(b), fp++) {
-                                const uint32_t src = c.first - O::ctz(b) * c.stride;
+                            O::for_each_bit(up, [&](uint32_t j) {
+                                const uint32_t src = c.first - j * c.stride;
                                 const uint64_t pr = c.tpass ? (((uint64_t)mg << 32) | src)
                                                             : (((uint64_t)src << 32) | mg);
                                 if (fp < p.fill_cap) p.fills[fp] = pr;
-                            }
+                                fp++;
+                            });
                         } else if (nf) {  // counts only: fills per row
                             if (c.tpass) {
                                 aadd64(&p.row_fills[mg], nf);
                             } else {
-                                for (M b = up; b; b = O::low_clear(b)) atomicAdd(&S.s_fill[O::ctz(b)], 1ull);
+                                O::for_each_bit(up, [&](uint32_t j) { atomicAdd(&S.s_fill[j], 1ull); });
                             }
                         }
                     }
