@@ -6,6 +6,7 @@
 // with_space_optimization/symbfact.cuh:2126) and has only a stub for
 // supernodes (with_space_optimization/with_supernodes/symbfact.cuh:1087-1101);
 // the outputs here are defined by the fill-path sets themselves.
+#include "gsofa_device.h"
 #include "gsofa_internal.h"
 
 namespace gsofa {
@@ -18,6 +19,19 @@ constexpr int SCAN_TILE = SB * SCAN_ITEMS;
 constexpr int SORT_LDS = 4096;   // longest segment sorted by LDS bitonic
 constexpr int BM_WORDS = 8192;   // 32 KB LDS bitmap for long segments
 constexpr uint32_t BM_BITS = BM_WORDS * 32u;
+constexpr int WSORT = 1024;      // longest segment sorted by one wave
+using dev::WAVE;
+
+// Fill-stream tiles: FT threads take FK entries each (FTILE per tile) and fold
+// them into an LDS table of FH slots keyed by (row, part), at most half full.
+constexpr int FT = 512;
+constexpr int FK = 4;
+constexpr int FTILE = FT * FK;
+constexpr int FH = 2 * FTILE;
+constexpr int FH_LOG2 = 12;
+constexpr int FGRID = 1024;      // persistent blocks (LDS allows 2-4 per CU)
+static_assert(FH == 1 << FH_LOG2, "table slots are a power of two");
+static_assert(FTILE <= 65536 && FH <= 65536, "slots and ranks pack into 16 bits each");
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t count, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
@@ -71,14 +85,87 @@ __device__ __forceinline__ uint64_t fills_in_stream(const FinalParams& f) {
     return pos < f.n_fills ? pos : f.n_fills;
 }
 
-__global__ void k_count_fills(FinalParams f) {
+// The LDS table of one stream tile: key (row << 1 | part) + 1, 0 when empty
+// (part 0: L, column below the row; part 1: U), and the slots in use.
+struct FTab {
+    uint32_t key[FH];
+    uint32_t cnt[FH];
+    uint16_t used[FTILE];
+    uint32_t nused[2];  // by tile parity: one is counted while the other is reset
+};
+
+__device__ __forceinline__ void ftab_init(FTab& T) {
+    for (uint32_t i = threadIdx.x; i < (uint32_t)FH; i += blockDim.x) {
+        T.key[i] = 0u;
+        T.cnt[i] = 0u;
+    }
+    if (threadIdx.x < 2) T.nused[threadIdx.x] = 0u;
+}
+
+// The slot of key, inserted if absent.  A tile holds at most FTILE distinct
+// keys and the table FH = 2 * FTILE slots, cleared after every tile, so the
+// probe always ends.
+__device__ __forceinline__ uint32_t ftab_slot(FTab& T, uint32_t key, uint32_t par) {
+    uint32_t s = (key * 2654435761u) >> (32 - FH_LOG2);
+    for (;;) {
+        const uint32_t cur = T.key[s];
+        if (cur == key) return s;
+        if (cur == 0u) {
+            const uint32_t prev = atomicCAS(&T.key[s], 0u, key);
+            if (prev == 0u) {
+                T.used[atomicAdd(&T.nused[par], 1u)] = (uint16_t)s;
+                return s;
+            }
+            if (prev == key) return s;
+        }
+        s = (s + 1u) & (uint32_t)(FH - 1);
+    }
+}
+
+// Key of a stream entry, or 0 for the ~0 entries a workgroup reserved and
+// left unused (row >= n).
+__device__ __forceinline__ uint32_t fill_key(uint64_t e, int32_t n) {
+    const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
+    if (row >= (uint32_t)n) return 0u;
+    return ((row << 1) | (col < row ? 0u : 1u)) + 1u;
+}
+
+// Per-row L and U fill counts.  Neighbouring stream entries share rows (one
+// source's fills over a relax batch, one target's fills over its slots), so
+// a tile's entries are counted per (row, part) in LDS first: one global
+// atomic per distinct key per tile instead of one per entry.
+__global__ __launch_bounds__(FT) void k_count_fills(FinalParams f) {
+    __shared__ FTab T;
+    const uint32_t t = threadIdx.x;
+    ftab_init(T);
+    __syncthreads();
     const uint64_t nf = fills_in_stream(f);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t e = f.fills[i];
-        const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
-        if (row >= (uint32_t)f.n) continue;  // reserved, unused (~0)
-        atomicAdd(col < row ? &f.cntL[row] : &f.cntU[row], 1u);
+    uint32_t par = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * FTILE; base < nf; base += (uint64_t)gridDim.x * FTILE) {
+        uint64_t e[FK];
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+            const uint64_t i = base + (uint64_t)k * FT + t;
+            e[k] = i < nf ? f.fills[i] : ~0ull;
+        }
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+            const uint32_t key = fill_key(e[k], f.n);
+            if (key) atomicAdd(&T.cnt[ftab_slot(T, key, par)], 1u);
+        }
+        // the other parity's count was last read before the previous barrier
+        if (t == 0) T.nused[par ^ 1u] = 0u;
+        __syncthreads();
+        const uint32_t nu = T.nused[par];
+        for (uint32_t u = t; u < nu; u += FT) {
+            const uint32_t sl = T.used[u];
+            const uint32_t k1 = T.key[sl] - 1u, row = k1 >> 1;
+            atomicAdd((k1 & 1u) ? &f.cntU[row] : &f.cntL[row], T.cnt[sl]);
+            T.key[sl] = 0u;
+            T.cnt[sl] = 0u;
+        }
+        __syncthreads();
+        par ^= 1u;
     }
 }
 
@@ -106,20 +193,61 @@ __global__ void k_scatter_rows(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
     }
 }
 
-__global__ void k_scatter_fills(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
+// Fill entries after A's: a tile's entries take local ranks per (row, part)
+// in LDS, one global atomic per distinct key reserves the run in the row, and
+// each entry lands at the run's start plus its rank.  Order within a row is
+// arbitrary (the segment sort follows).
+__global__ __launch_bounds__(FT) void k_scatter_fills(FinalParams f, int32_t* tmpL, int32_t* tmpU) {
+    __shared__ FTab T;
+    __shared__ int64_t rbase[FH];  // start of a slot's run, written after the tile's first barrier
+    const uint32_t t = threadIdx.x;
+    ftab_init(T);
+    __syncthreads();
     const uint64_t nf = fills_in_stream(f);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nf;
-         i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t e = f.fills[i];
-        const uint32_t row = (uint32_t)(e >> 32), col = (uint32_t)e;
-        if (row >= (uint32_t)f.n) continue;  // reserved, unused (~0)
-        if (col < row) {
-            const int64_t q = f.L_rowptr[row] + atomicAdd(&f.cntL[row], 1u);
-            if (q < f.L_cap) tmpL[q] = (int32_t)col;
-        } else {
-            const int64_t q = f.U_rowptr[row] + atomicAdd(&f.cntU[row], 1u);
-            if (q < f.U_cap) tmpU[q] = (int32_t)col;
+    uint32_t par = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * FTILE; base < nf; base += (uint64_t)gridDim.x * FTILE) {
+        uint64_t e[FK];
+        uint32_t sr[FK];  // slot << 16 | rank, or ~0 for no entry
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+            const uint64_t i = base + (uint64_t)k * FT + t;
+            e[k] = i < nf ? f.fills[i] : ~0ull;
         }
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+            const uint32_t key = fill_key(e[k], f.n);
+            sr[k] = ~0u;
+            if (key) {
+                const uint32_t sl = ftab_slot(T, key, par);
+                sr[k] = (sl << 16) | atomicAdd(&T.cnt[sl], 1u);
+            }
+        }
+        if (t == 0) T.nused[par ^ 1u] = 0u;
+        // also: every entry of the previous tile has read rbase
+        __syncthreads();
+        const uint32_t nu = T.nused[par];
+        for (uint32_t u = t; u < nu; u += FT) {
+            const uint32_t sl = T.used[u];
+            const uint32_t k1 = T.key[sl] - 1u, row = k1 >> 1;
+            const uint32_t c = T.cnt[sl];
+            rbase[sl] = (k1 & 1u) ? f.U_rowptr[row] + (int64_t)atomicAdd(&f.cntU[row], c)
+                                  : f.L_rowptr[row] + (int64_t)atomicAdd(&f.cntL[row], c);
+            T.key[sl] = 0u;
+            T.cnt[sl] = 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < FK; k++) {
+            if (sr[k] == ~0u) continue;
+            const uint32_t row = (uint32_t)(e[k] >> 32), col = (uint32_t)e[k];
+            const int64_t q = rbase[sr[k] >> 16] + (int64_t)(sr[k] & 0xffffu);
+            if (col < row) {
+                if (q < f.L_cap) tmpL[q] = (int32_t)col;
+            } else {
+                if (q < f.U_cap) tmpU[q] = (int32_t)col;
+            }
+        }
+        par ^= 1u;
     }
 }
 
@@ -140,99 +268,177 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t x, uint32_t* sh
     return incl - x;
 }
 
-// Sort each row segment (L part and U part separately) from tmp into the final
-// arrays.  Values within a segment are distinct, so a segment longer than the
-// LDS bitonic tile is sorted by setting bits of an LDS bitmap over successive
-// value windows and emitting them in order.
-// Rows ending beyond cap (entries allocated in src and dst) are left alone,
-// so a row-pointer total above the allocation cannot write out of bounds
-// (the host reports that case as an error).
-__global__ __launch_bounds__(SB) void k_sort_segments(const int64_t* rowptr, const int32_t* src,
-                                                      int32_t* dst, int32_t n, int64_t cap) {
+// Segments of one to WSORT entries, one wave each, no workgroup barriers:
+// up to 64 entries by a bitonic network over lanes (shuffles), longer ones by
+// a bitonic network over the wave's own LDS buffer.  Values within a segment
+// are distinct and below INT32_MAX, which pads.
+__global__ __launch_bounds__(SB) void k_sort_waves(const int64_t* rowptr, const int32_t* src, int32_t* dst,
+                                                   int32_t n, int64_t cap) {
+    constexpr int WPB = SB / WAVE;
+    __shared__ int32_t buf[WPB][WSORT];
+    const int lane = (int)__lane_id();
+    const int w = (int)__builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+    int32_t* s = buf[w];
+    for (int64_t r = (int64_t)blockIdx.x * WPB + w; r < n; r += (int64_t)gridDim.x * WPB) {
+        const int64_t b = rowptr[r], e = rowptr[r + 1];
+        const int64_t len = e - b;
+        if (len <= 0 || b < 0 || e > cap || len > WSORT) continue;
+        if (len <= WAVE) {
+            int32_t x = lane < len ? src[b + lane] : INT32_MAX;
+            for (int k = 2; k <= WAVE; k <<= 1) {
+                for (int j = k >> 1; j > 0; j >>= 1) {
+                    const int32_t y = __shfl_xor(x, j);
+                    const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+                    x = keep_min ? (x < y ? x : y) : (x > y ? x : y);
+                }
+            }
+            if (lane < len) dst[b + lane] = x;
+            continue;
+        }
+        int P = 2 * WAVE;
+        while (P < len) P <<= 1;
+        for (int i = lane; i < P; i += WAVE) s[i] = i < len ? src[b + i] : INT32_MAX;
+        dev::wave_sync();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int q = lane; q < P / 2; q += WAVE) {
+                    const int i = ((q & ~(j - 1)) << 1) | (q & (j - 1));  // pair (i, i + j)
+                    const int32_t a = s[i], c = s[i + j];
+                    if ((a > c) == ((i & k) == 0)) {
+                        s[i] = c;
+                        s[i + j] = a;
+                    }
+                }
+                dev::wave_sync();
+            }
+        }
+        for (int i = lane; i < len; i += WAVE) dst[b + i] = s[i];
+        dev::wave_sync();  // read out before the next segment overwrites the buffer
+    }
+}
+
+// One segment longer than WSORT by the whole block.  Its value range comes
+// first: a range of at most BM_WINDOWS windows of BM_BITS values (or a segment
+// longer than SORT_LDS) is sorted by setting bits of an LDS bitmap per window
+// (only the words the range covers) and emitting them in order; otherwise an
+// LDS bitonic network sorts it.  Called by every thread of the block.
+constexpr int64_t BM_WINDOWS = 4;
+__device__ void block_sort_segment(int64_t b, int64_t len, const int32_t* src, int32_t* dst, int32_t* s,
+                                   uint32_t* bm, uint32_t* sh, int32_t* red) {
+    const int t = threadIdx.x;
+    if (t == 0) {
+        red[0] = INT32_MAX;
+        red[1] = INT32_MIN;
+    }
+    __syncthreads();
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int64_t i = t; i < len; i += SB) {
+        const int32_t v = src[b + i];
+        lo = v < lo ? v : lo;
+        hi = v > hi ? v : hi;
+    }
+    atomicMin(&red[0], lo);
+    atomicMax(&red[1], hi);
+    __syncthreads();
+    lo = red[0];
+    hi = red[1];
+    __syncthreads();  // red is read before the next segment resets it
+    const int64_t range = (int64_t)hi - lo + 1;
+    if (len <= SORT_LDS && range > BM_WINDOWS * (int64_t)BM_BITS) {
+        int P = 2;
+        while (P < len) P <<= 1;
+        for (int i = t; i < P; i += SB) s[i] = i < len ? src[b + i] : INT32_MAX;
+        __syncthreads();
+        for (int k = 2; k <= P; k <<= 1) {
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = t; i < P; i += SB) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const int32_t a = s[i], c = s[ixj];
+                        const bool up = (i & k) == 0;
+                        if ((a > c) == up) {
+                            s[i] = c;
+                            s[ixj] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        for (int i = t; i < len; i += SB) dst[b + i] = s[i];
+        __syncthreads();
+        return;
+    }
+    int64_t written = 0;
+    for (int64_t wb = lo; wb <= hi; wb += BM_BITS) {
+        const int64_t span = (int64_t)hi - wb + 1;
+        const int nwords = span >= (int64_t)BM_BITS ? BM_WORDS : (int)((span + 31) >> 5);
+        for (int i = t; i < nwords; i += SB) bm[i] = 0u;
+        __syncthreads();
+        for (int64_t i = t; i < len; i += SB) {
+            const int64_t v = src[b + i];
+            if (v >= wb && v < wb + (int64_t)BM_BITS) {
+                const uint32_t o = (uint32_t)(v - wb);
+                atomicOr(&bm[o >> 5], 1u << (o & 31u));
+            }
+        }
+        __syncthreads();
+        // thread t takes words [w0, w1), consecutive so that its bits follow
+        // the bits of threads below it
+        const int wpt = (nwords + SB - 1) / SB;
+        const int w0 = t * wpt < nwords ? t * wpt : nwords;
+        const int w1 = w0 + wpt < nwords ? w0 + wpt : nwords;
+        uint32_t cnt = 0;
+        for (int w = w0; w < w1; w++) cnt += (uint32_t)__popc(bm[w]);
+        uint32_t total;
+        const uint32_t pos = block_excl_scan_u32(cnt, sh, &total);
+        int64_t o = b + written + pos;
+        for (int w = w0; w < w1; w++) {
+            uint32_t bits = bm[w];
+            while (bits) {
+                const uint32_t bp = (uint32_t)__ffs(bits) - 1u;
+                bits &= bits - 1u;
+                dst[o++] = (int32_t)(wb + (int64_t)w * 32 + bp);
+            }
+        }
+        written += total;
+        __syncthreads();
+    }
+}
+
+// Segments longer than WSORT: the first wave marks them in tiles of 64 rows,
+// the block sorts each.  Rows ending beyond cap (entries allocated in src and
+// dst) are left alone here and in k_sort_waves, so a row-pointer total above
+// the allocation cannot write out of bounds (the host reports that case as an
+// error).
+__global__ __launch_bounds__(SB) void k_sort_long(const int64_t* rowptr, const int32_t* src, int32_t* dst,
+                                                  int32_t n, int64_t cap) {
     __shared__ int32_t s[SORT_LDS];
     __shared__ uint32_t bm[BM_WORDS];
     __shared__ uint32_t sh[SB];
     __shared__ int32_t red[2];
+    __shared__ uint64_t longs;
     const int t = threadIdx.x;
-    for (int64_t r = blockIdx.x; r < n; r += gridDim.x) {
-        const int64_t b = rowptr[r], e = rowptr[r + 1];
-        const int64_t len = e - b;
-        if (len <= 0 || b < 0 || e > cap) continue;
-        if (len == 1) {
-            if (t == 0) dst[b] = src[b];
-            continue;
-        }
-        if (len <= SORT_LDS) {
-            int P = 2;
-            while (P < len) P <<= 1;
-            for (int i = t; i < P; i += SB) s[i] = i < len ? src[b + i] : INT32_MAX;
-            __syncthreads();
-            for (int k = 2; k <= P; k <<= 1) {
-                for (int j = k >> 1; j > 0; j >>= 1) {
-                    for (int i = t; i < P; i += SB) {
-                        const int ixj = i ^ j;
-                        if (ixj > i) {
-                            const int32_t a = s[i], c = s[ixj];
-                            const bool up = (i & k) == 0;
-                            if ((a > c) == up) {
-                                s[i] = c;
-                                s[ixj] = a;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                }
+    for (int64_t r0 = (int64_t)blockIdx.x * WAVE; r0 < n; r0 += (int64_t)gridDim.x * WAVE) {
+        if (t < WAVE) {
+            const int64_t r = r0 + t;
+            bool lng = false;
+            if (r < n) {
+                const int64_t b = rowptr[r], e = rowptr[r + 1];
+                lng = e - b > WSORT && b >= 0 && e <= cap;
             }
-            for (int i = t; i < len; i += SB) dst[b + i] = s[i];
-            __syncthreads();
-            continue;
-        }
-        // long segment: value range, then bitmap windows
-        if (t == 0) {
-            red[0] = INT32_MAX;
-            red[1] = INT32_MIN;
+            const uint64_t m = __ballot(lng);
+            if (t == 0) longs = m;
         }
         __syncthreads();
-        int32_t lo = INT32_MAX, hi = INT32_MIN;
-        for (int64_t i = t; i < len; i += SB) {
-            const int32_t v = src[b + i];
-            lo = v < lo ? v : lo;
-            hi = v > hi ? v : hi;
+        uint64_t m = longs;
+        while (m) {
+            const int64_t r = r0 + (__ffsll((unsigned long long)m) - 1);
+            m &= m - 1ull;
+            const int64_t b = rowptr[r];
+            block_sort_segment(b, rowptr[r + 1] - b, src, dst, s, bm, sh, red);
         }
-        atomicMin(&red[0], lo);
-        atomicMax(&red[1], hi);
-        __syncthreads();
-        lo = red[0];
-        hi = red[1];
-        int64_t written = 0;
-        for (int64_t wb = lo; wb <= hi; wb += BM_BITS) {
-            for (int i = t; i < BM_WORDS; i += SB) bm[i] = 0u;
-            __syncthreads();
-            for (int64_t i = t; i < len; i += SB) {
-                const int64_t v = src[b + i];
-                if (v >= wb && v < wb + (int64_t)BM_BITS) {
-                    const uint32_t o = (uint32_t)(v - wb);
-                    atomicOr(&bm[o >> 5], 1u << (o & 31u));
-                }
-            }
-            __syncthreads();
-            constexpr int WPT = BM_WORDS / SB;  // words per thread
-            uint32_t cnt = 0;
-            for (int w = 0; w < WPT; w++) cnt += (uint32_t)__popc(bm[t * WPT + w]);
-            uint32_t total;
-            uint32_t pos = block_excl_scan_u32(cnt, sh, &total);
-            int64_t o = b + written + pos;
-            for (int w = 0; w < WPT; w++) {
-                uint32_t bits = bm[t * WPT + w];
-                while (bits) {
-                    const uint32_t bp = (uint32_t)__ffs(bits) - 1u;
-                    bits &= bits - 1u;
-                    dst[o++] = (int32_t)(wb + (int64_t)(t * WPT + w) * 32 + bp);
-                }
-            }
-            written += total;
-            __syncthreads();
-        }
+        __syncthreads();  // every thread has read longs
     }
 }
 
@@ -499,7 +705,8 @@ hipError_t exclusive_scan_u32_to_i64(const uint32_t* in, int64_t* out, int64_t n
 
 hipError_t launch_count(const FinalParams& f, hipStream_t s) {
     hipLaunchKernelGGL(k_count_rows, grid_for(f.n), dim3(SB), 0, s, f);
-    if (f.n_fills) hipLaunchKernelGGL(k_count_fills, grid_for((int64_t)f.n_fills), dim3(SB), 0, s, f);
+    if (f.n_fills)
+        hipLaunchKernelGGL(k_count_fills, grid_for((int64_t)f.n_fills, FTILE, FGRID), dim3(FT), 0, s, f);
     hipLaunchKernelGGL(k_row_fill, grid_for(f.n), dim3(SB), 0, s, f);
     return hipGetLastError();
 }
@@ -511,15 +718,16 @@ hipError_t launch_scatter(const FinalParams& f, hipStream_t s) {
     // f.L_col / f.U_col here are the *unsorted* staging arrays
     hipLaunchKernelGGL(k_scatter_rows, grid_for(f.n), dim3(SB), 0, s, f, f.L_col, f.U_col);
     if (f.n_fills)
-        hipLaunchKernelGGL(k_scatter_fills, grid_for((int64_t)f.n_fills), dim3(SB), 0, s, f, f.L_col,
-                           f.U_col);
+        hipLaunchKernelGGL(k_scatter_fills, grid_for((int64_t)f.n_fills, FTILE, FGRID), dim3(FT), 0, s, f,
+                           f.L_col, f.U_col);
     return hipGetLastError();
 }
 
 hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32_t* dst, int32_t n, int64_t cap,
                                 hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_sort_segments, grid_for(n, 1, 1 << 20), dim3(SB), 0, s, rowptr, src, dst, n, cap);
+    hipLaunchKernelGGL(k_sort_waves, grid_for(n, SB / WAVE, 8192), dim3(SB), 0, s, rowptr, src, dst, n, cap);
+    hipLaunchKernelGGL(k_sort_long, grid_for(n, WAVE, 4096), dim3(SB), 0, s, rowptr, src, dst, n, cap);
     return hipGetLastError();
 }
 
