@@ -20,6 +20,7 @@ constexpr int SORT_LDS = 4096;   // longest segment sorted by LDS bitonic
 constexpr int BM_WORDS = 8192;   // 32 KB LDS bitmap for long segments
 constexpr uint32_t BM_BITS = BM_WORDS * 32u;
 constexpr int WSORT = 1024;      // longest segment sorted by one wave
+constexpr int LTILE = 16;        // rows scanned per step for longer segments
 using dev::WAVE;
 
 // Fill-stream tiles: FT threads take FK entries each (FTILE per tile) and fold
@@ -406,24 +407,26 @@ __device__ void block_sort_segment(int64_t b, int64_t len, const int32_t* src, i
     }
 }
 
-// Segments longer than WSORT: the first wave marks them in tiles of 64 rows,
-// the block sorts each.  Rows ending beyond cap (entries allocated in src and
+// Segments longer than WSORT: the first wave marks them in tiles of LTILE
+// rows (small, so that runs of long rows spread over blocks), the block sorts
+// each.  Rows ending beyond cap (entries allocated in src and
 // dst) are left alone here and in k_sort_waves, so a row-pointer total above
 // the allocation cannot write out of bounds (the host reports that case as an
 // error).
 __global__ __launch_bounds__(SB) void k_sort_long(const int64_t* rowptr, const int32_t* src, int32_t* dst,
                                                   int32_t n, int64_t cap) {
-    __shared__ int32_t s[SORT_LDS];
-    __shared__ uint32_t bm[BM_WORDS];
+    static_assert(SORT_LDS <= BM_WORDS, "the bitonic buffer shares the bitmap's words");
+    __shared__ uint32_t bm[BM_WORDS];  // bitmap windows, or the bitonic buffer
     __shared__ uint32_t sh[SB];
     __shared__ int32_t red[2];
     __shared__ uint64_t longs;
+    int32_t* s = reinterpret_cast<int32_t*>(bm);
     const int t = threadIdx.x;
-    for (int64_t r0 = (int64_t)blockIdx.x * WAVE; r0 < n; r0 += (int64_t)gridDim.x * WAVE) {
+    for (int64_t r0 = (int64_t)blockIdx.x * LTILE; r0 < n; r0 += (int64_t)gridDim.x * LTILE) {
         if (t < WAVE) {
             const int64_t r = r0 + t;
             bool lng = false;
-            if (r < n) {
+            if (t < LTILE && r < n) {
                 const int64_t b = rowptr[r], e = rowptr[r + 1];
                 lng = e - b > WSORT && b >= 0 && e <= cap;
             }
@@ -727,7 +730,7 @@ hipError_t launch_sort_segments(const int64_t* rowptr, const int32_t* src, int32
                                 hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sort_waves, grid_for(n, SB / WAVE, 8192), dim3(SB), 0, s, rowptr, src, dst, n, cap);
-    hipLaunchKernelGGL(k_sort_long, grid_for(n, WAVE, 4096), dim3(SB), 0, s, rowptr, src, dst, n, cap);
+    hipLaunchKernelGGL(k_sort_long, grid_for(n, LTILE, 8192), dim3(SB), 0, s, rowptr, src, dst, n, cap);
     return hipGetLastError();
 }
 
