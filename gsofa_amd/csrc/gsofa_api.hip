@@ -14,9 +14,11 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <dlfcn.h>
+#include <sys/mman.h>
 
 #include "gsofa.h"
 #include "gsofa_internal.h"
@@ -534,10 +536,16 @@ int ensure_cols(gsofa_plan* P, int64_t nl, int64_t nu) {
 
 // Device-to-host copies of large results go through pinned staging buffers
 // (a process-wide pool, kept until exit): the DMA of one chunk overlaps the
-// host copy of the previous one into the caller's (pageable) array, where a
-// plain copy to pageable memory moves a few GB/s.
-constexpr size_t kStageBytes = 32u << 20;
-constexpr size_t kStageMin = 8u << 20;  // smaller copies go direct
+// host copy of the previous one into the caller's (pageable) array.  That host
+// copy is what limits the rate: it first touches fresh pages (a few GB/s per
+// thread), so a large copy is split over up to kCopyParts threads, each with
+// two staging buffers of its own, and result arrays are allocated on 2 MB
+// boundaries with transparent huge pages requested (host_array).
+constexpr size_t kStageBytes = 16u << 20;
+constexpr size_t kStageMin = 8u << 20;    // smaller copies go direct
+constexpr size_t kPartMin = 64u << 20;    // bytes per copy thread, at least
+constexpr int kCopyParts = 4;
+constexpr size_t kHugePage = 2u << 20;
 std::mutex g_stage_mu;
 std::vector<void*> g_stage_free;
 
@@ -564,21 +572,10 @@ void stage_put(void* p) {
     g_stage_free.push_back(p);
 }
 
-// dst <- src (device), complete on return.  Work queued on s before the call
-// is ordered before the copy.
-hipError_t copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
-    if (bytes == 0) return hipSuccess;
-    void* b[2] = {nullptr, nullptr};
-    if (bytes >= kStageMin) {
-        b[0] = stage_get();
-        b[1] = stage_get();
-    }
-    if (!b[0] || !b[1]) {  // small, or no pinned memory: one direct copy
-        stage_put(b[0]);
-        stage_put(b[1]);
-        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
-        return e == hipSuccess ? hipStreamSynchronize(s) : e;
-    }
+// dst <- src through the staging buffers b0, b1: chunk k + 1's DMA runs while
+// chunk k is copied out.  Every DMA has ended when this returns.
+hipError_t copy_staged(void* dst, const void* src, size_t bytes, hipStream_t s, void* b0, void* b1) {
+    void* b[2] = {b0, b1};
     hipEvent_t ev[2] = {nullptr, nullptr};
     hipError_t e = hipEventCreateWithFlags(&ev[0], hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ev[1], hipEventDisableTiming);
@@ -600,9 +597,68 @@ hipError_t copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
     if (e != hipSuccess) (void)hipStreamSynchronize(s);  // no copy left in flight into the buffers
     for (hipEvent_t x : ev)
         if (x) (void)hipEventDestroy(x);
-    stage_put(b[0]);
-    stage_put(b[1]);
     return e;
+}
+
+// dst <- src (device), complete on return.  Work queued on s before the call
+// is ordered before the copy.
+hipError_t copy_d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    int parts = 0;
+    void* b[2 * kCopyParts] = {};
+    if (bytes >= kStageMin) {
+        const size_t want = std::min<size_t>(kCopyParts, std::max<size_t>(1, bytes / kPartMin));
+        for (; parts < (int)want; parts++) {
+            b[2 * parts] = stage_get();
+            b[2 * parts + 1] = stage_get();
+            if (!b[2 * parts] || !b[2 * parts + 1]) {
+                stage_put(b[2 * parts]);
+                stage_put(b[2 * parts + 1]);
+                b[2 * parts] = b[2 * parts + 1] = nullptr;
+                break;
+            }
+        }
+    }
+    if (parts == 0) {  // small, or no pinned memory: one direct copy
+        hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s);
+        return e == hipSuccess ? hipStreamSynchronize(s) : e;
+    }
+    // parts of whole staging chunks; the last takes the remainder
+    const size_t chunks = (bytes + kStageBytes - 1) / kStageBytes;
+    const size_t per = (chunks + parts - 1) / parts * kStageBytes;
+    std::vector<hipError_t> err((size_t)parts, hipSuccess);
+    std::vector<std::thread> th;
+    auto part = [&](int i) {
+        const size_t off = (size_t)i * per;
+        if (off >= bytes) return;
+        err[(size_t)i] = copy_staged((char*)dst + off, (const char*)src + off, std::min(per, bytes - off), s,
+                                     b[2 * i], b[2 * i + 1]);
+    };
+    for (int i = 1; i < parts; i++) {
+        try {
+            th.emplace_back(part, i);
+        } catch (...) {  // no thread: this one copies that part too
+            part(i);
+        }
+    }
+    part(0);
+    for (std::thread& t : th) t.join();
+    for (int i = 0; i < 2 * parts; i++) stage_put(b[i]);
+    for (hipError_t x : err)
+        if (x != hipSuccess) return x;
+    return hipSuccess;
+}
+
+// A host result array of `bytes`, released with free(): large ones start on
+// a 2 MB boundary with transparent huge pages requested, so that their first
+// touch (the staged copy above) faults once per 2 MB rather than per 4 KB.
+void* host_array(size_t bytes) {
+    if (bytes < kStageMin) return malloc(std::max<size_t>(bytes, 1));
+    const size_t len = (bytes + kHugePage - 1) / kHugePage * kHugePage;
+    void* p = nullptr;
+    if (posix_memalign(&p, kHugePage, len) != 0) return nullptr;
+    (void)madvise(p, len, MADV_HUGEPAGE);  // advice only: small pages if refused
+    return p;
 }
 
 float ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -1947,8 +2003,8 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
     r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
     r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
     r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(n + 1));
-    r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(P->nnzL, 1));
-    r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(P->nnzU, 1));
+    r->L_col = (int32_t*)host_array(sizeof(int32_t) * (size_t)std::max<int64_t>(P->nnzL, 1));
+    r->U_col = (int32_t*)host_array(sizeof(int32_t) * (size_t)std::max<int64_t>(P->nnzU, 1));
     if (!r->row_fill || !r->L_rowptr || !r->U_rowptr || !r->L_col || !r->U_col) {
         gsofa_result_free(r);
         return GSOFA_ERR_ALLOC;
@@ -2056,8 +2112,8 @@ int gsofa_plan_fetch_matrix(gsofa_plan* P, int32_t index, gsofa_result* r) {
     if (lu) {
         r->L_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
         r->U_rowptr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(nb + 1));
-        r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
-        r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
+        r->L_col = (int32_t*)host_array(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
+        r->U_col = (int32_t*)host_array(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
     }
     if (G.row_te) {
         r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
@@ -2577,8 +2633,8 @@ int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
         r->U_rowptr[i] -= u0;
     }
     for (int32_t i = 0; i < nb; i++) r->fill_count += (uint64_t)r->row_fill[i];
-    r->L_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
-    r->U_col = (int32_t*)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
+    r->L_col = (int32_t*)host_array(sizeof(int32_t) * (size_t)std::max<int64_t>(l1 - l0, 1));
+    r->U_col = (int32_t*)host_array(sizeof(int32_t) * (size_t)std::max<int64_t>(u1 - u0, 1));
     if (!r->L_col || !r->U_col) {
         gsofa_result_free(r);
         return GSOFA_ERR_ALLOC;
