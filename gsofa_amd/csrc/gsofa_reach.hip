@@ -75,14 +75,20 @@ constexpr int NT_WIDE = 1024, NH_WIDE = 11264;
 constexpr int NT_PAIR = 512, NH_PAIR = 5632;
 constexpr uint32_t FULLI = 0xFFFFFFFFu;
 constexpr uint32_t PASS_T = 0x80000000u;  // chunk stride bit: the A^T pass
+// Lane groups: 256 per workgroup, each taking 12 edges of its vertex per
+// step: 4 lanes x 3 edges at 1024 threads, 2 lanes x 6 edges at 512 (256
+// groups measured faster than 128 or 512 at either size).  GSOFA_RG and
+// GSOFA_RE, when defined at build time, set one shape for both.
 #ifndef GSOFA_RG
-#define GSOFA_RG 4
+#define GSOFA_RG 0
 #endif
 #ifndef GSOFA_RE
-#define GSOFA_RE 3
+#define GSOFA_RE 0
 #endif
-constexpr int RG = GSOFA_RG;  // lanes per frontier vertex
-constexpr int RE = GSOFA_RE;  // edges per lane and step
+template <int NT>
+constexpr int lanes_per_vertex() { return GSOFA_RG > 0 ? GSOFA_RG : NT / 256; }
+template <int NT>
+constexpr int edges_per_lane() { return GSOFA_RE > 0 ? GSOFA_RE : 12 / lanes_per_vertex<NT>(); }
 
 template <class M>
 struct MaskOps;
@@ -449,6 +455,7 @@ template <int RB, int H, int G, int E, class M>
 __global__ __launch_bounds__(RB) void k_reach_blk(RchParams p) {
     static_assert(H / 2 + RB <= H, "insert overshoot stays within the entry list");
     static_assert(H <= 65536, "entries are 16-bit slot numbers");
+    static_assert(WAVE % G == 0 && G < 32 && E >= 1 && E < 32, "lane groups tile a wave; a lane's edges fit its pending bits");
     using O = MaskOps<M>;
     using VW = typename O::V;
     __shared__ RLds<M, H> S;
@@ -699,11 +706,13 @@ bool rch_threads_valid(int threads) { return threads == NT_WIDE || threads == NT
 namespace {
 template <int NT, class M>
 const void* kptr() {
-    return reinterpret_cast<const void*>(&k_reach_blk<NT, tab_slots<M>(NT), RG, RE, M>);
+    return reinterpret_cast<const void*>(
+        &k_reach_blk<NT, tab_slots<M>(NT), lanes_per_vertex<NT>(), edges_per_lane<NT>(), M>);
 }
 template <int NT, class M>
 void klaunch(const RchParams& p, hipStream_t s) {
-    hipLaunchKernelGGL((k_reach_blk<NT, tab_slots<M>(NT), RG, RE, M>), dim3(p.nblocks), dim3(NT), 0, s, p);
+    hipLaunchKernelGGL((k_reach_blk<NT, tab_slots<M>(NT), lanes_per_vertex<NT>(), edges_per_lane<NT>(), M>),
+                       dim3(p.nblocks), dim3(NT), 0, s, p);
 }
 template <int NT>
 const void* kptr_bits(int bits) {
