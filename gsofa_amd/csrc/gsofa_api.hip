@@ -193,6 +193,18 @@ const char* kErr[] = {"ok",
             return GSOFA_ERR_HIP;                                                       \
         }                                                                               \
     } while (0)
+// The same, for code that has queued copies into host memory on stream s:
+// they end before the error returns, since the caller may free that memory.
+#define HIPCHK_OUT(x, s)                                                                \
+    do {                                                                                \
+        hipError_t e_ = (x);                                                            \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "gsofa: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), \
+                    __FILE__, __LINE__);                                                \
+            (void)hipStreamSynchronize(s);                                              \
+            return GSOFA_ERR_HIP;                                                       \
+        }                                                                               \
+    } while (0)
 
 template <class T>
 int dalloc(T** p, size_t count) {
@@ -1571,9 +1583,9 @@ static int build_structure(gsofa_plan* P, uint64_t n_fills, int32_t N, int32_t g
         P->totals_known = false;
     } else {
         int64_t tot[2] = {0, 0};
-        HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(&tot[0], P->d_Lrp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(&tot[1], P->d_Urp + n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         P->nnzL = tot[0];
         P->nnzU = tot[1];
         P->totals_known = true;
@@ -1594,9 +1606,9 @@ static int build_structure(gsofa_plan* P, uint64_t n_fills, int32_t N, int32_t g
 static int read_totals(gsofa_plan* P) {
     if (P->totals_known || P->n == 0) return GSOFA_OK;
     int64_t tot[2] = {0, 0};
-    HIPCHK(hipMemcpyAsync(&tot[0], P->d_Lrp + P->n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-    HIPCHK(hipMemcpyAsync(&tot[1], P->d_Urp + P->n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipMemcpyAsync(&tot[0], P->d_Lrp + P->n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+    HIPCHK_OUT(hipMemcpyAsync(&tot[1], P->d_Urp + P->n, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     if (tot[0] > P->Lcap || tot[1] > P->Ucap) return GSOFA_ERR_STATE;  // cannot happen: the bounds hold
     P->nnzL = tot[0];
     P->nnzU = tot[1];
@@ -1671,8 +1683,8 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
             HIPCHK(hipMemcpyAsync(P->d_ctrl, &snap, sizeof(Ctrl), hipMemcpyHostToDevice, P->stream));
             int rc = run_share(P, gid, si);
             if (rc) return rc;
-            HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
-            HIPCHK(hipStreamSynchronize(P->stream));
+            HIPCHK_OUT(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream), P->stream);
+            HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
             if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
             if (P->stats.fill_pos <= P->fill_cap) break;
             dfree(P->d_fills);
@@ -1687,19 +1699,19 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
         SpillShare& h = P->sp[(size_t)gid];
         h.Lcol.resize((size_t)std::max<int64_t>(P->nnzL, 1));
         h.Ucol.resize((size_t)std::max<int64_t>(P->nnzU, 1));
-        HIPCHK(hipMemcpyAsync(lrp.data(), P->d_Lrp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipMemcpyAsync(urp.data(), P->d_Urp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
-                              P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(lrp.data(), P->d_Lrp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(urp.data(), P->d_Urp, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
         if (P->nnzL)
-            HIPCHK(hipMemcpyAsync(h.Lcol.data(), P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL,
-                                  hipMemcpyDeviceToHost, P->stream));
+            HIPCHK_OUT(hipMemcpyAsync(h.Lcol.data(), P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL,
+                                      hipMemcpyDeviceToHost, P->stream), P->stream);
         if (P->nnzU)
-            HIPCHK(hipMemcpyAsync(h.Ucol.data(), P->d_Ucol, sizeof(int32_t) * (size_t)P->nnzU,
-                                  hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(rowfill.data(), P->d_rowfill, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+            HIPCHK_OUT(hipMemcpyAsync(h.Ucol.data(), P->d_Ucol, sizeof(int32_t) * (size_t)P->nnzU,
+                                      hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(rowfill.data(), P->d_rowfill, sizeof(int64_t) * (size_t)n, hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         // the share's rows are the only non-empty ones: keep their offsets
         h.Loff.assign(1, 0);
         h.Uoff.assign(1, 0);
@@ -1732,7 +1744,7 @@ static int execute_spill(gsofa_plan* P, const std::vector<int32_t>& shares) {
         P->share_max = std::max(P->share_max, t);
     }
     HIPCHK(hipEventRecord(P->ev[1], P->stream));
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     P->ms = ev_ms(P->ev[0], P->ev[1]);
     P->executed = true;
     return GSOFA_OK;
@@ -1785,7 +1797,7 @@ static int queue_bounded(gsofa_plan* P, const std::vector<int32_t>& shares, int3
         if (!rc && P->supernodes) rc = build_supernodes(P, nullptr);
         if (rc) return rc;
     }
-    HIPCHK(hipMemcpyAsync(stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK_OUT(hipMemcpyAsync(stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream), P->stream);
     if (!P->capturing) HIPCHK(hipEventRecord(P->ev[1], P->stream));
     return GSOFA_OK;
 }
@@ -1866,7 +1878,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
         HIPCHK(hipEventRecord(P->ev[0], P->stream));
         HIPCHK(hipGraphLaunch(P->graph_exec, P->stream));
         HIPCHK(hipEventRecord(P->ev[1], P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         P->stats = *P->h_ctrl;
         if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
         if (!P->structure || P->stats.fill_pos <= P->fill_cap) {
@@ -1886,7 +1898,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
         // back; the host waits once, at the end
         int rc = queue_bounded(P, shares, N, all_rows, &P->stats);
         if (rc) return rc;
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
         if (!P->structure || P->stats.fill_pos <= P->fill_cap) {
             finish_timing(P, shares.size());
@@ -1907,8 +1919,8 @@ int gsofa_plan_execute(gsofa_plan* P) {
             int rc = run_share(P, shares[si], si);
             if (rc) return rc;
         }
-        HIPCHK(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(&P->stats, P->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         if (P->stats.abort) return GSOFA_ERR_DEVICE_TIMEOUT;
         if (!P->structure || P->stats.fill_pos <= P->fill_cap) break;
         // fill stream outgrew its reserve: grow to the exact size and rerun
@@ -1927,7 +1939,7 @@ int gsofa_plan_execute(gsofa_plan* P) {
         }
     }
     HIPCHK(hipEventRecord(P->ev[1], P->stream));
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     finish_timing(P, shares.size());
     P->executed = true;
     return GSOFA_OK;
@@ -1965,7 +1977,7 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         }
         return GSOFA_OK;
     }
-    HIPCHK(hipSetDevice(P->device));
+    HIPCHK_OUT(hipSetDevice(P->device), P->stream);
     if (P->structure && !P->spilled) {
         const int rc = read_totals(P);
         if (rc) return rc;
@@ -1976,10 +1988,10 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         if (P->mode == GSOFA_MODE_REACH) {
             r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
             if (!r->row_fill) return GSOFA_ERR_ALLOC;
-            HIPCHK(hipMemcpyAsync(r->row_fill, P->d_row_fk, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
-                                  P->stream));
-            HIPCHK(fetch_mat_totals(P));
-            HIPCHK(hipStreamSynchronize(P->stream));
+            HIPCHK_OUT(hipMemcpyAsync(r->row_fill, P->d_row_fk, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                                      P->stream), P->stream);
+            HIPCHK_OUT(fetch_mat_totals(P), P->stream);
+            HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
             return GSOFA_OK;
         }
         if (P->mode != GSOFA_MODE_BLOCK || P->hybrid) return GSOFA_OK;
@@ -1990,11 +2002,13 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
             gsofa_result_free(r);
             return GSOFA_ERR_ALLOC;
         }
-        HIPCHK(hipMemcpyAsync(r->row_fill, P->d_row_fk, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(r->row_te, P->d_row_te, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->row_fill, P->d_row_fk, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(r->row_te, P->d_row_te, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         const int32_t N = std::max(1, P->opt.n_virtual_gpu), id = P->opt.virtual_gpu_id;
         for (int32_t i = 0; i < n; i++)
             if (id >= 0 && i % N != id) r->row_fill[i] = r->row_te[i] = r->row_reach[i] = -1;
@@ -2010,7 +2024,8 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
         return GSOFA_ERR_ALLOC;
     }
     if (!P->spilled)
-        HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->row_fill, P->d_rowfill, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
     if (P->mode == GSOFA_MODE_BLOCK && !P->hybrid) {
         r->row_te = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
         r->row_reach = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
@@ -2018,9 +2033,10 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
             gsofa_result_free(r);
             return GSOFA_ERR_ALLOC;
         }
-        HIPCHK(hipMemcpyAsync(r->row_te, P->d_row_te, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
-                              P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->row_te, P->d_row_te, sizeof(int64_t) * n, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(r->row_reach, P->d_row_reach, sizeof(int64_t) * n, hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
     }
     if (P->spilled) {
         // rows in order, each from the share that built it (row i: share i mod N)
@@ -2043,21 +2059,23 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
             r->U_rowptr[i + 1] = uo;
         }
     } else {
-        HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(copy_d2h(r->L_col, P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL, P->stream));
-        HIPCHK(copy_d2h(r->U_col, P->d_Ucol, sizeof(int32_t) * (size_t)P->nnzU, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->L_rowptr, P->d_Lrp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(r->U_rowptr, P->d_Urp, sizeof(int64_t) * (n + 1), hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(copy_d2h(r->L_col, P->d_Lcol, sizeof(int32_t) * (size_t)P->nnzL, P->stream), P->stream);
+        HIPCHK_OUT(copy_d2h(r->U_col, P->d_Ucol, sizeof(int32_t) * (size_t)P->nnzU, P->stream), P->stream);
     }
     if (r->row_te) {
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         for (int32_t i = 0; i < n; i++)
             if (r->row_fill[i] < 0) r->row_te[i] = r->row_reach[i] = -1;
     }
-    HIPCHK(fetch_mat_totals(P));
+    HIPCHK_OUT(fetch_mat_totals(P), P->stream);
     if (P->supernodes) {
         int32_t ns = 0;
-        HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         r->n_super = ns;
         r->super_start = (int32_t*)malloc(sizeof(int32_t) * (size_t)(ns + 1));
         r->supno = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
@@ -2065,11 +2083,12 @@ int gsofa_plan_fetch(gsofa_plan* P, gsofa_result* r) {
             gsofa_result_free(r);
             return GSOFA_ERR_ALLOC;
         }
-        HIPCHK(hipMemcpyAsync(r->super_start, P->d_sstart, sizeof(int32_t) * (ns + 1), hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipMemcpyAsync(r->supno, P->d_supno, sizeof(int32_t) * n, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->super_start, P->d_sstart, sizeof(int32_t) * (ns + 1), hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(r->supno, P->d_supno, sizeof(int32_t) * n, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
     }
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     return GSOFA_OK;
 }
 
@@ -2202,12 +2221,12 @@ int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
     const int32_t n = P->n;
     h->n = n;
     int32_t ns = 0;
-    HIPCHK(hipSetDevice(P->device));
+    HIPCHK_OUT(hipSetDevice(P->device), P->stream);
     const int trc = read_totals(P);
     if (trc) return trc;
     if (n > 0) {
-        HIPCHK(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(&ns, P->d_nsuper, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     }
     h->n_super = ns;
     DevScratch T;  // device scratch for this call only
@@ -2217,20 +2236,20 @@ int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
         T.get(&d_lrow, (size_t)P->nnzL) || T.get(&d_urow, (size_t)P->nnzU))
         return GSOFA_ERR_ALLOC;
     // L and U by columns; the staging arrays of the structure pass are free now
-    HIPCHK(transpose_structure(P->d_Lrp, P->d_Lcol, P->nnzL, n, P->d_cntL, d_xl, P->d_scan, P->d_Ltmp, d_lrow,
-                               P->stream));
-    HIPCHK(transpose_structure(P->d_Urp, P->d_Ucol, P->nnzU, n, P->d_cntU, d_xu, P->d_scan, P->d_Utmp, d_urow,
-                               P->stream));
+    HIPCHK_OUT(transpose_structure(P->d_Lrp, P->d_Lcol, P->nnzL, n, P->d_cntL, d_xl, P->d_scan, P->d_Ltmp, d_lrow,
+                                   P->stream), P->stream);
+    HIPCHK_OUT(transpose_structure(P->d_Urp, P->d_Ucol, P->nnzU, n, P->d_cntU, d_xu, P->d_scan, P->d_Utmp, d_urow,
+                                   P->stream), P->stream);
     // supernode subscripts
     int64_t nsub = 0;
     int32_t* d_sub = nullptr;
     if (ns > 0) {
-        HIPCHK(supernode_sub_count(P->d_sstart, ns, P->d_Urp, P->d_cntL, P->stream));
-        HIPCHK(exclusive_scan_u32_to_i64(P->d_cntL, d_xs, ns, P->d_scan, P->stream));
-        HIPCHK(hipMemcpyAsync(&nsub, d_xs + ns, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(supernode_sub_count(P->d_sstart, ns, P->d_Urp, P->d_cntL, P->stream), P->stream);
+        HIPCHK_OUT(exclusive_scan_u32_to_i64(P->d_cntL, d_xs, ns, P->d_scan, P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(&nsub, d_xs + ns, sizeof(int64_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         if (T.get(&d_sub, (size_t)nsub)) return GSOFA_ERR_ALLOC;
-        HIPCHK(supernode_sub_write(P->d_sstart, ns, P->d_Urp, P->d_Ucol, d_xs, d_sub, P->stream));
+        HIPCHK_OUT(supernode_sub_write(P->d_sstart, ns, P->d_Urp, P->d_Ucol, d_xs, d_sub, P->stream), P->stream);
     }
     h->xsup = (int32_t*)malloc(sizeof(int32_t) * ((size_t)ns + 1));
     h->supno = (int32_t*)malloc(sizeof(int32_t) * std::max<size_t>((size_t)n, 1));
@@ -2246,22 +2265,28 @@ int gsofa_plan_fetch_supernodal(gsofa_plan* P, gsofa_supernodal* h) {
     }
     h->xsup[0] = 0;
     if (n > 0) {
-        HIPCHK(hipMemcpyAsync(h->xsup, P->d_sstart, sizeof(int32_t) * ((size_t)ns + 1), hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipMemcpyAsync(h->supno, P->d_supno, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipMemcpyAsync(h->xlcol, d_xl, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(h->xucol, d_xu, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(h->xsup, P->d_sstart, sizeof(int32_t) * ((size_t)ns + 1), hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(h->supno, P->d_supno, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(h->xlcol, d_xl, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(h->xucol, d_xu, sizeof(int64_t) * ((size_t)n + 1), hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
     }
     if (P->nnzL)
-        HIPCHK(hipMemcpyAsync(h->lrow, d_lrow, sizeof(int32_t) * (size_t)P->nnzL, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(h->lrow, d_lrow, sizeof(int32_t) * (size_t)P->nnzL, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
     if (P->nnzU)
-        HIPCHK(hipMemcpyAsync(h->urow, d_urow, sizeof(int32_t) * (size_t)P->nnzU, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(h->urow, d_urow, sizeof(int32_t) * (size_t)P->nnzU, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
     if (ns > 0) {
-        HIPCHK(hipMemcpyAsync(h->xsub, d_xs, sizeof(int64_t) * ((size_t)ns + 1), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(h->sub, d_sub, sizeof(int32_t) * (size_t)nsub, hipMemcpyDeviceToHost, P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(h->xsub, d_xs, sizeof(int64_t) * ((size_t)ns + 1), hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(h->sub, d_sub, sizeof(int32_t) * (size_t)nsub, hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
     }
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     return GSOFA_OK;
 }
 
@@ -2440,7 +2465,7 @@ int arena_load(gsofa_plan* P, const PackedMatrix* const* mats, int32_t count, in
                              8 * mx(c.L, nl) + 8 * mx(c.U, nu) + 8 * mx(c.scan, scan_tmp_words(n) + 2 * (size_t)n + 8) +
                              reach_bytes((uint32_t)std::max<size_t>(c.nb, nb), (uint32_t)std::max<size_t>(c.wg, W), 64);
         if (keep > budget && P->held > 0) {
-            HIPCHK(hipStreamSynchronize(P->stream));
+            HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
             release_device(P);
             P->cap = gsofa_plan::Caps();
             P->sn_cap = 0;
@@ -2584,7 +2609,7 @@ int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
     if (!P || !P->arena || !r || index < 0 || index >= P->n_mats) return GSOFA_ERR_INVALID;
     if (!P->executed) return GSOFA_ERR_STATE;
     memset(r, 0, sizeof(*r));
-    HIPCHK(hipSetDevice(P->device));
+    HIPCHK_OUT(hipSetDevice(P->device), P->stream);
     const int64_t b0 = P->h_mat_base[(size_t)index], b1 = P->h_mat_base[(size_t)index + 1];
     const int32_t nb = (int32_t)(b1 - b0);
     r->n = nb;
@@ -2597,16 +2622,17 @@ int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
     r->passes = P->passes;
     r->n_super = -1;
     unsigned long long te = 0, reach = 0;
-    HIPCHK(hipMemcpyAsync(&te, P->d_mat_te + index, sizeof(te), hipMemcpyDeviceToHost, P->stream));
-    HIPCHK(hipMemcpyAsync(&reach, P->d_mat_reach + index, sizeof(reach), hipMemcpyDeviceToHost, P->stream));
+    HIPCHK_OUT(hipMemcpyAsync(&te, P->d_mat_te + index, sizeof(te), hipMemcpyDeviceToHost, P->stream), P->stream);
+    HIPCHK_OUT(hipMemcpyAsync(&reach, P->d_mat_reach + index, sizeof(reach), hipMemcpyDeviceToHost, P->stream),
+               P->stream);
     r->te_alg = te;
     r->reach = reach;
     r->row_fill = (int64_t*)malloc(sizeof(int64_t) * (size_t)std::max(nb, 1));
     if (!r->row_fill) return GSOFA_ERR_ALLOC;
     if (!P->structure) {  // counts only: the kernels' per-row tallies
-        HIPCHK(hipMemcpyAsync(r->row_fill, P->d_row_fk + b0, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->row_fill, P->d_row_fk + b0, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         r->te_alg = te;
         r->reach = reach;
         for (int32_t i = 0; i < nb; i++) r->fill_count += (uint64_t)r->row_fill[i];
@@ -2618,13 +2644,13 @@ int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
         gsofa_result_free(r);
         return GSOFA_ERR_ALLOC;
     }
-    HIPCHK(hipMemcpyAsync(r->L_rowptr, P->d_Lrp + b0, sizeof(int64_t) * ((size_t)nb + 1), hipMemcpyDeviceToHost,
-                          P->stream));
-    HIPCHK(hipMemcpyAsync(r->U_rowptr, P->d_Urp + b0, sizeof(int64_t) * ((size_t)nb + 1), hipMemcpyDeviceToHost,
-                          P->stream));
-    HIPCHK(hipMemcpyAsync(r->row_fill, P->d_rowfill + b0, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost,
-                          P->stream));
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipMemcpyAsync(r->L_rowptr, P->d_Lrp + b0, sizeof(int64_t) * ((size_t)nb + 1), hipMemcpyDeviceToHost,
+                              P->stream), P->stream);
+    HIPCHK_OUT(hipMemcpyAsync(r->U_rowptr, P->d_Urp + b0, sizeof(int64_t) * ((size_t)nb + 1), hipMemcpyDeviceToHost,
+                              P->stream), P->stream);
+    HIPCHK_OUT(hipMemcpyAsync(r->row_fill, P->d_rowfill + b0, sizeof(int64_t) * (size_t)nb, hipMemcpyDeviceToHost,
+                              P->stream), P->stream);
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     r->te_alg = te;
     r->reach = reach;
     const int64_t l0 = r->L_rowptr[0], l1 = r->L_rowptr[nb], u0 = r->U_rowptr[0], u1 = r->U_rowptr[nb];
@@ -2640,16 +2666,17 @@ int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
         return GSOFA_ERR_ALLOC;
     }
     if (b0) {  // global to the matrix's own column numbers
-        HIPCHK(launch_add(P->d_Lcol + l0, (uint64_t)(l1 - l0), -(int32_t)b0, P->stream));
-        HIPCHK(launch_add(P->d_Ucol + u0, (uint64_t)(u1 - u0), -(int32_t)b0, P->stream));
+        HIPCHK_OUT(launch_add(P->d_Lcol + l0, (uint64_t)(l1 - l0), -(int32_t)b0, P->stream), P->stream);
+        HIPCHK_OUT(launch_add(P->d_Ucol + u0, (uint64_t)(u1 - u0), -(int32_t)b0, P->stream), P->stream);
     }
-    HIPCHK(copy_d2h(r->L_col, P->d_Lcol + l0, sizeof(int32_t) * (size_t)(l1 - l0), P->stream));
-    HIPCHK(copy_d2h(r->U_col, P->d_Ucol + u0, sizeof(int32_t) * (size_t)(u1 - u0), P->stream));
+    HIPCHK_OUT(copy_d2h(r->L_col, P->d_Lcol + l0, sizeof(int32_t) * (size_t)(l1 - l0), P->stream), P->stream);
+    HIPCHK_OUT(copy_d2h(r->U_col, P->d_Ucol + u0, sizeof(int32_t) * (size_t)(u1 - u0), P->stream), P->stream);
     if (P->supernodes && nb > 0) {
         int32_t s0 = 0, s1 = 0;
-        HIPCHK(hipMemcpyAsync(&s0, P->d_supno + b0, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(&s1, P->d_supno + b1 - 1, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(&s0, P->d_supno + b0, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(&s1, P->d_supno + b1 - 1, sizeof(int32_t), hipMemcpyDeviceToHost, P->stream),
+                   P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         // supernodes never cross matrices: (i-1) is never in L_i at a boundary
         r->n_super = s1 - s0 + 1;
         r->super_start = (int32_t*)malloc(sizeof(int32_t) * ((size_t)r->n_super + 1));
@@ -2658,16 +2685,16 @@ int arena_fetch(gsofa_plan* P, int32_t index, gsofa_result* r) {
             gsofa_result_free(r);
             return GSOFA_ERR_ALLOC;
         }
-        HIPCHK(hipMemcpyAsync(r->super_start, P->d_sstart + s0, sizeof(int32_t) * (size_t)r->n_super,
-                              hipMemcpyDeviceToHost, P->stream));
-        HIPCHK(hipMemcpyAsync(r->supno, P->d_supno + b0, sizeof(int32_t) * (size_t)nb, hipMemcpyDeviceToHost,
-                              P->stream));
-        HIPCHK(hipStreamSynchronize(P->stream));
+        HIPCHK_OUT(hipMemcpyAsync(r->super_start, P->d_sstart + s0, sizeof(int32_t) * (size_t)r->n_super,
+                                  hipMemcpyDeviceToHost, P->stream), P->stream);
+        HIPCHK_OUT(hipMemcpyAsync(r->supno, P->d_supno + b0, sizeof(int32_t) * (size_t)nb, hipMemcpyDeviceToHost,
+                                  P->stream), P->stream);
+        HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
         for (int32_t s = 0; s < r->n_super; s++) r->super_start[s] -= (int32_t)b0;
         r->super_start[r->n_super] = nb;
         for (int32_t i = 0; i < nb; i++) r->supno[i] -= s0;
     }
-    HIPCHK(hipStreamSynchronize(P->stream));
+    HIPCHK_OUT(hipStreamSynchronize(P->stream), P->stream);
     return GSOFA_OK;
 }
 
